@@ -1,0 +1,160 @@
+"""Benchmark: MLUPS and HBM-roofline fraction of the two-component D3Q27 TSLB
+step on one MI355X.
+
+Workload: the emulsion configuration -- 512^3 periodic, two components,
+200 non-overlapping droplets with radii in [8, 16], surface tension 0.01,
+recolouring beta 0.7, linear forcing, FP32 compute and FP32 moment storage.
+One step = colour-gradient normals + fused moment-space collide/pull-stream over
+the whole lattice, inputs resident in HBM.
+
+Prints one JSON line.  `roofline` is for the dominant (fused) kernel:
+algorithmic bytes per launch (88 B per lattice update: 11 FP32 fields read
+once and written once) over its mean launch time, measured with HIP events on
+the library's stream inside the timed region.  `cpu_baseline` times the CPU
+oracle (the reference's population-push algorithm, FP32) on a slab of the same
+workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & % HBM roofline, 2C D3Q27 512³; closure-LES speedup at fixed error"
+HBM_PEAK_GBS = 8000.0
+FIELDS_2C = 11
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, case, budget_s=15.0, threads=None):
+    """Time the FP32 CPU oracle (population push, as the reference) on a slab
+    n x n x nzs of the emulsion workload, periodic."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    threads = threads or min(os.cpu_count() or 1, 16)
+    po.set_threads(threads)
+    nzs = max(4, min(32, n // 16))
+    f = {k: np.ascontiguousarray(v[:nzs]) for k, v in case.fields.items()}
+    kw = case.lattice
+    o = po.Oracle(n, n, nzs, ncomp=2, tau=kw["tau"], sigma=kw["sigma"], sharp_c=kw["sharp_c"],
+                  lin_a=kw["lin_force"], periodic=(1, 1, 1), dtype=np.float32)
+    from entropicthreadsafe_amd.configs import rand_noseeded_grid
+    vel = {}
+    for name, amp, l in case.noise:   # the same seeded noise as the GPU run, first nzs planes
+        r = rand_noseeded_grid(n, n, nzs, l)
+        vel[name] = (f[name] + np.float32(amp) * (np.float32(2) * r - np.float32(1))).astype(np.float32)
+    o.init(f["rho"], vel["u"], vel["v"], vel["w"], f["phi"], True)
+    t0 = time.perf_counter()
+    o.step(1)
+    t1 = time.perf_counter() - t0
+    steps = max(2, int(budget_s / max(t1, 1e-3)))
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    mlups = n * n * nzs * steps / dt / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP32 population-push oracle, emulsion slab {n}x{n}x{nzs} periodic, "
+                      f"{steps} steps in {dt:.1f} s"}
+
+
+def load_traffic(path):
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="lattice edge (512 for the named workload)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1 (the multi-GPU slab "
+              "exchange is a design, not a built path)", file=sys.stderr)
+        sys.exit(2)
+
+    from entropicthreadsafe_amd import configs
+    from entropicthreadsafe_amd.tslb import Lattice
+
+    n = args.n
+    t0 = time.time()
+    case = configs.emulsion(n=n)
+    log(f"bench: emulsion {n}^3 set up in {time.time() - t0:.1f} s ({case.notes})")
+    nodes = n ** 3
+    with Lattice(n, n, n, device=0, **case.lattice) as lat:
+        case.apply(lat)
+        log(f"bench: state on device ({lat.device_bytes() / 2**30:.2f} GiB)")
+        lat.step(args.warmup)
+        lat.sync()
+        r0 = lat.reduce()
+        lat.timing_enable(True)
+        lat.sync()
+        t0 = time.perf_counter()
+        lat.step(args.steps)
+        lat.sync()
+        wall = time.perf_counter() - t0
+        fused_ms, fused_n = lat.timing_read(0)
+        norm_ms, norm_n = lat.timing_read(1)
+        lat.timing_enable(False)
+        r1 = lat.reduce()
+    log(f"bench: {args.steps} steps in {wall:.3f} s; fused {fused_ms / max(fused_n, 1):.3f} ms x {fused_n}, "
+        f"normals {norm_ms / max(norm_n, 1):.3f} ms x {norm_n}; mass drift {(r1[0] - r0[0]) / r0[0]:.2e}, "
+        f"phi drift {(r1[1] - r0[1]) / max(r0[1], 1e-30):.2e}")
+
+    mlups = nodes * args.steps / wall / 1e6
+    bytes_per_launch = 8.0 * FIELDS_2C * nodes          # read + write of 11 FP32 fields
+    t_fused = fused_ms / max(fused_n, 1) / 1e3
+    achieved = bytes_per_launch / t_fused / 1e9
+    t_step = (fused_ms + norm_ms) / max(fused_n, 1) / 1e3
+    traffic = load_traffic(os.path.join(ROOT, "profiles", "fused_2c_512_traffic.json"))
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded droplets and velocity noise)",
+        "config": {"workload": f"emulsion-2c-{n}^3", "nx": n, "ny": n, "nz": n, "components": 2,
+                   "storage": "fp32", "droplets": case.notes, "tau": case.lattice["tau"],
+                   "sigma": case.lattice["sigma"], "sharp_c": case.lattice["sharp_c"],
+                   "lin_force": case.lattice["lin_force"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_fused<2>", "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "mean_launch_ms": round(t_fused * 1e3, 4)},
+        "step_roofline_frac": round(bytes_per_launch / t_step / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n, case, args.cpu_budget)
+        except Exception as e:  # the baseline is reported, never the product
+            log(f"bench: cpu baseline failed: {e}")
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

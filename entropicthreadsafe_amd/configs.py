@@ -1,0 +1,215 @@
+"""Workload set-up: synthetic initial states of the TSLB configurations.
+
+Each builder returns a `Case`: the Lattice keyword arguments, the initial
+rho/u/v/w/phi fields (numpy, shape (nz, ny, nx)), an optional isfluid map and a
+list of seeded-noise terms `(field, amplitude, l)` meaning
+field += amp * (2 r - 1), r = rand_noseeded(i, j, k, l) at 1-based (i, j, k).
+Noise is applied on the device by `apply()` (tslb_seed_noise) or on the host by
+`host_fields()`; both give the same values.
+
+Configurations (all seeds through the reference's counter-based generator,
+TSLB_newBCS/vars_single_3D_module.f90:102-165):
+  tgv       3D Taylor-Green vortex, one component, periodic
+  droplet   static droplet, two components, periodic
+  channel   pressure-driven channel, one component, walls at y, additive force
+  emulsion  droplets in linearly forced turbulence, two components, periodic
+"""
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+
+# ---------------------------------------------------------------------------
+# vectorised counter-based generator (host side, small and medium grids)
+# ---------------------------------------------------------------------------
+def rand_noseeded_grid(nx, ny, nz, l):
+    """rand_noseeded(i, j, k, l) for every 1-based node, shape (nz, ny, nx)."""
+    k, j, i = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1),
+                          indexing="ij")
+    isub = (i % 178 + 1).astype(np.int64)
+    jsub = (j % 178 + 1).astype(np.int64)
+    ksub = (k % 178 + 1).astype(np.int64)
+    lsub = int(l) % 169
+    out = []
+    for _ in range(2):
+        s = np.zeros(isub.shape, dtype=np.float32)
+        t = np.float32(0.5)
+        for _ in range(24):
+            msub = ((isub * jsub) % 179 * ksub) % 179
+            isub, jsub, ksub = jsub, ksub, msub
+            lsub = (53 * lsub + 1) % 169
+            s = np.where((lsub * msub) % 64 >= 32, s + t, s).astype(np.float32)
+            t = np.float32(t * np.float32(0.5))
+        out.append(s)
+    uni = (out[0] - out[1]).astype(np.float32)
+    uni = np.where(uni < 0, uni + np.float32(1.0), uni).astype(np.float32)
+    cc = np.float32(362436.0 / 16777216.0)
+    cd = np.float32(7654321.0 / 16777216.0)
+    cm = np.float32(16777213.0 / 16777216.0)
+    csub = np.float32(cc - cd)
+    if csub < 0:
+        csub = np.float32(csub + cm)
+    uni = (uni - csub).astype(np.float32)
+    uni = np.where(uni < 0, uni + np.float32(1.0), uni).astype(np.float32)
+    return uni
+
+
+@dataclass
+class Case:
+    name: str
+    shape: Tuple[int, int, int]            # (nz, ny, nx)
+    lattice: Dict                           # Lattice(...) keyword arguments
+    fields: Dict[str, np.ndarray]           # rho, u, v, w, phi
+    noise: List[Tuple[str, float, int]] = field(default_factory=list)
+    isfluid: Optional[np.ndarray] = None
+    prepush_phi: bool = True
+    notes: str = ""
+
+    def host_fields(self):
+        """Initial fields with the seeded noise applied on the host."""
+        f = {k: v.astype(np.float32).copy() for k, v in self.fields.items()}
+        nz, ny, nx = self.shape
+        for name, amp, l in self.noise:
+            r = rand_noseeded_grid(nx, ny, nz, l)
+            f[name] = (f[name] + np.float32(amp) * (np.float32(2.0) * r - np.float32(1.0))).astype(np.float32)
+        return f
+
+    def apply(self, lat):
+        """Upload the initial state to a Lattice (noise seeded on the device)."""
+        if self.isfluid is not None:
+            lat.set_isfluid(self.isfluid)
+        for name in ("rho", "u", "v", "w", "phi"):
+            lat.upload_field(name, self.fields[name])
+        for name, amp, l in self.noise:
+            lat.seed_noise(name, amp, l)
+        lat.init_equilibrium(prepush_phi=self.prepush_phi)
+
+
+def _grid(nx, ny, nz):
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return i, j, k
+
+
+def tgv(n=64, u0=0.05, re=1600.0, length=None):
+    """3D Taylor-Green vortex, one component, periodic n^3.
+
+    nu = u0 L / Re with L = n (the length scale the workload names), so
+    tau = 0.5 + 3 nu; x = i - 1 (0-based index), k = 2 pi / n.
+    """
+    L = float(n if length is None else length)
+    nu = u0 * L / re
+    tau = 0.5 + 3.0 * nu
+    i, j, k = _grid(n, n, n)
+    kk = 2.0 * np.pi / n
+    u = u0 * np.sin(kk * i) * np.cos(kk * j) * np.cos(kk * k)
+    v = -u0 * np.cos(kk * i) * np.sin(kk * j) * np.cos(kk * k)
+    z = np.zeros_like(u)
+    return Case("tgv", (n, n, n), dict(nx=n, ny=n, nz=n, ncomp=1, tau=tau, periodic=(1, 1, 1)),
+                dict(rho=np.ones_like(u), u=u, v=v, w=z, phi=z.copy()),
+                notes=f"nu={nu:.6g} tau={tau:.6g}")
+
+
+def droplet(n=256, radius=40.0, sigma=0.02, beta=0.7, noise=1e-3, tau=1.0):
+    """Static droplet of component A (phi=1) in B, two components, periodic n^3.
+
+    Sphere of the given radius centred at node (n/2, n/2, n/2) (1-based); seeded
+    velocity noise with l = 11, 22, 33; sharpening sharp_c = 3 beta (the
+    reference's 0.3*3 reads as beta/cs^2, recursive_2c_TSLB3D.f90:28).
+    """
+    i, j, k = _grid(n, n, n)
+    c = n / 2.0
+    r2 = (i + 1 - c) ** 2 + (j + 1 - c) ** 2 + (k + 1 - c) ** 2
+    phi = (r2 <= radius ** 2).astype(np.float64)
+    z = np.zeros_like(phi)
+    return Case("droplet", (n, n, n),
+                dict(nx=n, ny=n, nz=n, ncomp=2, tau=tau, sigma=sigma, sharp_c=3.0 * beta,
+                     periodic=(1, 1, 1)),
+                dict(rho=np.ones_like(phi), u=z, v=z.copy(), w=z.copy(), phi=phi),
+                noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)])
+
+
+def channel(nx=512, ny=256, nz=256, re_tau=180.0, u_tau=0.005, pert=0.05, ub_over_utau=15.6):
+    """Pressure-driven channel, one component: x, z periodic, halfway
+    bounce-back walls just outside y = 1 and y = ny (half height h = ny/2).
+
+    nu = u_tau h / Re_tau, body force F_x = u_tau^2 / h (additive,
+    recursive_TSLB3D.f90:571-707).  Start: parabolic profile with bulk velocity
+    ub_over_utau * u_tau plus seeded perturbations of pert * U_b on u, v, w.
+    """
+    h = ny / 2.0
+    nu = u_tau * h / re_tau
+    tau = 0.5 + 3.0 * nu
+    fx = u_tau ** 2 / h
+    i, j, k = _grid(nx, ny, nz)
+    yy = j + 0.5                      # distance from the lower wall
+    ub = ub_over_utau * u_tau
+    u = 1.5 * ub * (yy / h) * (2.0 - yy / h)
+    z = np.zeros_like(u)
+    amp = pert * ub
+    return Case("channel", (nz, ny, nx),
+                dict(nx=nx, ny=ny, nz=nz, ncomp=1, tau=tau, force=(fx, 0.0, 0.0),
+                     force_mode=0, periodic=(1, 0, 1)),
+                dict(rho=np.ones_like(u), u=u, v=z, w=z.copy(), phi=z.copy()),
+                noise=[("u", amp, 101), ("v", amp, 102), ("w", amp, 103)],
+                notes=f"nu={nu:.6g} tau={tau:.6g} Fx={fx:.6g}")
+
+
+def place_droplets(n, ndrop, rmin, rmax, seed_l=7):
+    """Non-overlapping droplets by seeded rejection sampling (periodic box).
+
+    Candidate m uses rand_noseeded(m, 1..4, 1, seed_l) for centre and radius.
+    """
+    from . import tslb as _t  # the generator of the library (host C)
+    drops = []
+    m = 0
+    while len(drops) < ndrop and m < 200000:
+        m += 1
+        cx = _t.rand_noseeded(m, 1, 1, seed_l) * n
+        cy = _t.rand_noseeded(m, 2, 1, seed_l) * n
+        cz = _t.rand_noseeded(m, 3, 1, seed_l) * n
+        r = rmin + (rmax - rmin) * _t.rand_noseeded(m, 4, 1, seed_l)
+        ok = True
+        for (x, y, z, rr) in drops:
+            d = np.array([cx - x, cy - y, cz - z])
+            d = d - n * np.round(d / n)
+            if np.sqrt((d ** 2).sum()) < r + rr + 2.0:
+                ok = False
+                break
+        if ok:
+            drops.append((cx, cy, cz, r))
+    return drops
+
+
+def rasterize_droplets(n, drops, dtype=np.float32):
+    phi = np.zeros((n, n, n), dtype=dtype)
+    for (cx, cy, cz, r) in drops:
+        lo = [int(np.floor(c - r)) - 1 for c in (cx, cy, cz)]
+        hi = [int(np.ceil(c + r)) + 2 for c in (cx, cy, cz)]
+        ax = [np.arange(lo[a], hi[a]) for a in range(3)]
+        X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
+        inside = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 <= r * r
+        xi, yi, zi = X[inside] % n, Y[inside] % n, Z[inside] % n
+        phi[zi, yi, xi] = 1
+    return phi
+
+
+def emulsion(n=512, ndrop=200, rmin=8.0, rmax=16.0, sigma=0.01, beta=0.7, tau=0.51,
+             lin_force=2.8e-5, noise=1e-2):
+    """Emulsion in linearly forced homogeneous turbulence, two components,
+    periodic n^3: ndrop non-overlapping droplets with radii U[rmin, rmax],
+    linear forcing u_eq += A u (velocity shift, as the reference's forcing
+    enters the two-component equilibrium, lb_kernels_module.f90:769-771),
+    seeded velocity noise.  A = 2.8e-5 with tau = 0.51 targets Re_lambda ~ 60
+    for the linear-forcing steady state (integral scale ~0.19 n).
+    """
+    drops = place_droplets(n, ndrop, rmin, rmax)
+    phi = rasterize_droplets(n, drops)
+    z = np.zeros((n, n, n), dtype=np.float32)
+    vol = float(phi.sum()) / n ** 3
+    return Case("emulsion", (n, n, n),
+                dict(nx=n, ny=n, nz=n, ncomp=2, tau=tau, sigma=sigma, sharp_c=3.0 * beta,
+                     lin_force=lin_force, periodic=(1, 1, 1)),
+                dict(rho=np.ones_like(z), u=z, v=z, w=z, phi=phi),
+                noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)],
+                notes=f"{len(drops)} droplets, volume fraction {vol:.4f}")

@@ -1,0 +1,1399 @@
+// tslb.hip -- MI355X (gfx950) TSLB library: HIP kernels and the C ABI of
+// include/tslb.h.
+//
+// State: SoA moment fields on the device, Fortran (i fastest) order, two
+// buffers (source / destination of a step).  Field 0 holds rho-1 (shifted
+// density), so that the FP32 sums of the fused kernel keep their precision;
+// the host interface converts to rho.
+//
+// One step (reference: recursive_2c_TSLB3D.f90:344-840) is, in moment space,
+//   m_{t+1}(y) = M[ f_l(y) ],   f_l(y) = R_l(m_t(y - c_l)),
+// where R_l is the recursive-regularized post-collision population
+// (feq_l(rho,u_eq) + (1-omega) w_l a^neq_l(Pi,u) [+ forcing]) and M takes the
+// moments (rho, u, Pi^neq; phi from the D3Q7 phase populations).  Bounce-back
+// replaces R_l(m(y-c_l)) by R_opp(l)(m(y)) when y-c_l is solid or beyond a wall.
+// Because the reference's collide kernel reads only node-local moments and
+// writes each destination once, this restatement is exact.
+//
+// Kernels
+//   k_fused     LDS-tiled 2.5D collide + pull-stream.  A 64x8 block marches a
+//               z-chunk; per source plane it evaluates the 27 (+7) populations
+//               of every node of the (64+2)x(8+2) footprint once, scatters the
+//               24 (+4) in-plane-moving ones through LDS, and each destination
+//               column gathers its populations and accumulates moment sums for
+//               three destination planes in registers.
+//   k_normals   colour gradient / normals (two-component runs).
+//   k_prepush   initial one-step advection of phi (reference writes g pushed).
+//   k_links     per-node bounce-back link masks from an isfluid map.
+//   k_zplane_bc moment-space inlet/outlet planes.
+//   k_reduce*   FP64 diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/resource.h>
+#include <sys/sysinfo.h>
+
+#include <string>
+#include <vector>
+
+#include "tslb.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Lattice tables (TSLB_newBCS/vars_single_3D_module.f90:21-43)
+// ---------------------------------------------------------------------------
+constexpr int kCX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0,
+                         0, 1, -1, -1, 1, 1, -1, 1, -1, -1, 1, 1, -1};
+constexpr int kCY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1,
+                         -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, -1, 1};
+constexpr int kCZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1,
+                         1, 1, -1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1};
+constexpr int kOPP[27] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14,
+                          13, 16, 15, 18, 17, 20, 19, 22, 21, 24, 23, 26, 25};
+
+__host__ __device__ constexpr float wgt(int l) {
+  return l == 0 ? (float)(8.0 / 27.0)
+                : (l <= 6 ? (float)(2.0 / 27.0) : (l <= 18 ? (float)(1.0 / 54.0) : (float)(1.0 / 216.0)));
+}
+
+// direction with velocity (cx,cy,cz)
+__host__ __device__ constexpr int dir_of(int cx, int cy, int cz) {
+  int r = -1;
+  for (int l = 0; l < 27; l++)
+    if (kCX[l] == cx && kCY[l] == cy && kCZ[l] == cz) r = l;
+  return r;
+}
+
+// LDS slot of a direction that moves in the x-y plane (all but 0, 5, 6)
+__host__ __device__ constexpr int qslot(int l) { return l <= 4 ? l - 1 : l - 3; }
+
+// 27-bit masks of the directions whose source y - c_l lies beyond a face
+__host__ __device__ constexpr uint32_t dirmask(int axis, int sign) {
+  uint32_t m = 0;
+  for (int l = 0; l < 27; l++) {
+    int c = axis == 0 ? kCX[l] : (axis == 1 ? kCY[l] : kCZ[l]);
+    if (c == sign) m |= 1u << l;
+  }
+  return m;
+}
+constexpr uint32_t kAllDirs = (1u << 27) - 2u;  // bits 1..26
+constexpr uint32_t kSolid = 1u << 31;
+
+// tile geometry of k_fused
+constexpr int TX = 64, TY = 8;
+constexpr int SW = TX + 2, SH = TY + 2, NSLOT = SW * SH;
+constexpr int NRING = 2 * SW + 2 * TY;
+constexpr int NTHR = TX * TY;
+
+struct Params {
+  int nx, ny, nz;
+  int px, py, pz;  // periodic flags
+  int ncomp;
+  int shift;       // velocity-shift forcing
+  float om1;       // 1 - omega
+  float sigma, sharp_c;
+  float fx, fy, fz, lin_a;
+  int z_chunk, tiles_x, tiles_y, nzc;
+};
+
+struct Fields {
+  float *f[TSLB_NFIELDS];
+};
+struct CFields {
+  const float *f[TSLB_NFIELDS];
+};
+
+__device__ __forceinline__ int wrapc(int g, int n) {
+  int r = g % n;
+  return r < 0 ? r + n : r;
+}
+
+// coordinate of a source along one axis; false beyond a non-periodic face
+__device__ __forceinline__ bool src_coord(int g, int n, int per, int &out) {
+  if (g >= 0 && g < n) {
+    out = g;
+    return true;
+  }
+  if (!per) return false;
+  out = wrapc(g, n);
+  return true;
+}
+
+template <int C>
+__device__ __forceinline__ float cm(float a) {
+  if constexpr (C == 0) return 0.f;
+  else if constexpr (C > 0) return a;
+  else return -a;
+}
+// c . (a, b, c) with only the non-zero components
+template <int X, int Y, int Z>
+__device__ __forceinline__ float cdot(float a, float b, float c) {
+  if constexpr (X != 0 && Y != 0 && Z != 0) return cm<X>(a) + cm<Y>(b) + cm<Z>(c);
+  else if constexpr (X != 0 && Y != 0) return cm<X>(a) + cm<Y>(b);
+  else if constexpr (X != 0 && Z != 0) return cm<X>(a) + cm<Z>(c);
+  else if constexpr (Y != 0 && Z != 0) return cm<Y>(b) + cm<Z>(c);
+  else if constexpr (X != 0) return cm<X>(a);
+  else if constexpr (Y != 0) return cm<Y>(b);
+  else if constexpr (Z != 0) return cm<Z>(c);
+  else return 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// Post-collision populations of one node, shifted by the weights:
+//   h_l = f*_l - w_l,  f*_l = feq_l(rho,u_eq) + (1-omega) w_l a^neq_l(Pi,u) [+F]
+// (recursive_2c_TSLB3D.f90:637-806, recursive_TSLB3D.f90:556-707).  Evaluated
+// per pair (l, opp(l)) as w_l (E +- O) with E, O the even and odd parts in c:
+//   E/w = drho - 3/2 rho u_eq^2 - Ts/3 + 9/2 rho (c.u_eq)^2 + c.Ps.c
+//   O/w = c.u_eq (3 rho - 9/2 rho u_eq^2 + 9/2 rho (c.u_eq)^2)
+//         + c.u (3 c.Ps.c - Ts) - 2 c.(Ps u) + 3 c.F
+// with Ps = 9/2 (1-omega) Pi^neq and Ts = tr Ps.
+// ---------------------------------------------------------------------------
+struct NodeC {
+  float K0, A, C1, Ts;
+  float ux, uy, uz, sx, sy, sz;  // u and u_eq
+  float pxx, pyy, pzz, pxy2, pxz2, pyz2;  // Ps, off-diagonals doubled
+  float qx, qy, qz;                       // Ps u
+  float f3x, f3y, f3z;                    // 3 F (additive forcing)
+};
+
+template <int L, bool SHIFT, bool ADD>
+__device__ __forceinline__ void pair_pop(const NodeC &n, float *h) {
+  constexpr int X = kCX[L], Y = kCY[L], Z = kCZ[L];
+  const float cus = cdot<X, Y, Z>(n.sx, n.sy, n.sz);
+  const float cu = SHIFT ? cdot<X, Y, Z>(n.ux, n.uy, n.uz) : cus;
+  // c.Ps.c
+  float cpc;
+  {
+    float d = 0.f;
+    bool any = false;
+    if constexpr (X != 0) { d = n.pxx; any = true; }
+    if constexpr (Y != 0) { d = any ? d + n.pyy : n.pyy; any = true; }
+    if constexpr (Z != 0) { d = any ? d + n.pzz : n.pzz; any = true; }
+    if constexpr (X != 0 && Y != 0) d = d + cm<X * Y>(n.pxy2);
+    if constexpr (X != 0 && Z != 0) d = d + cm<X * Z>(n.pxz2);
+    if constexpr (Y != 0 && Z != 0) d = d + cm<Y * Z>(n.pyz2);
+    cpc = d;
+  }
+  const float cq = cdot<X, Y, Z>(n.qx, n.qy, n.qz);
+  const float cus2 = cus * cus;
+  const float E = fmaf(n.A, cus2, n.K0) + cpc;
+  float O = cus * fmaf(n.A, cus2, n.C1);
+  O = fmaf(cu, fmaf(3.f, cpc, -n.Ts), O);
+  O = fmaf(-2.f, cq, O);
+  if constexpr (ADD) O = O + cdot<X, Y, Z>(n.f3x, n.f3y, n.f3z);
+  constexpr float w = wgt(L);
+  h[L] = w * (E + O);
+  h[kOPP[L]] = w * (E - O);
+}
+
+template <bool SHIFT, bool ADD>
+__device__ __forceinline__ void all_pops(const NodeC &n, float *h) {
+  h[0] = wgt(0) * n.K0;
+  pair_pop<1, SHIFT, ADD>(n, h);
+  pair_pop<3, SHIFT, ADD>(n, h);
+  pair_pop<5, SHIFT, ADD>(n, h);
+  pair_pop<7, SHIFT, ADD>(n, h);
+  pair_pop<9, SHIFT, ADD>(n, h);
+  pair_pop<11, SHIFT, ADD>(n, h);
+  pair_pop<13, SHIFT, ADD>(n, h);
+  pair_pop<15, SHIFT, ADD>(n, h);
+  pair_pop<17, SHIFT, ADD>(n, h);
+  pair_pop<19, SHIFT, ADD>(n, h);
+  pair_pop<21, SHIFT, ADD>(n, h);
+  pair_pop<23, SHIFT, ADD>(n, h);
+  pair_pop<25, SHIFT, ADD>(n, h);
+}
+
+// Per-node constants.  m = {drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz}; (esx,esy,esz)
+// is the velocity shift (CSF + body force + linear forcing) when SHIFT.
+__device__ __forceinline__ NodeC node_consts(const float *m, float esx, float esy, float esz,
+                                             const Params &p, bool shift, bool add) {
+  NodeC n;
+  const float drho = m[0];
+  const float rho = 1.f + drho;
+  n.ux = m[1];
+  n.uy = m[2];
+  n.uz = m[3];
+  n.sx = shift ? n.ux + esx : n.ux;
+  n.sy = shift ? n.uy + esy : n.uy;
+  n.sz = shift ? n.uz + esz : n.uz;
+  const float uu = n.sx * n.sx + n.sy * n.sy + n.sz * n.sz;
+  const float B = 4.5f * p.om1;
+  n.pxx = B * m[4];
+  n.pyy = B * m[5];
+  n.pzz = B * m[6];
+  const float pxy = B * m[7], pxz = B * m[8], pyz = B * m[9];
+  n.pxy2 = 2.f * pxy;
+  n.pxz2 = 2.f * pxz;
+  n.pyz2 = 2.f * pyz;
+  n.Ts = n.pxx + n.pyy + n.pzz;
+  n.A = 4.5f * rho;
+  n.K0 = fmaf(-1.5f * rho, uu, drho) - n.Ts * (1.f / 3.f);
+  n.C1 = fmaf(-n.A, uu, 3.f * rho);
+  n.qx = n.pxx * n.ux + pxy * n.uy + pxz * n.uz;
+  n.qy = pxy * n.ux + n.pyy * n.uy + pyz * n.uz;
+  n.qz = pxz * n.ux + pyz * n.uy + n.pzz * n.uz;
+  if (add) {
+    n.f3x = 3.f * p.fx;
+    n.f3y = 3.f * p.fy;
+    n.f3z = 3.f * p.fz;
+  } else {
+    n.f3x = n.f3y = n.f3z = 0.f;
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// Colour-gradient stencil, grouped as the reference (recursive_2c_TSLB3D.f90:
+// 555-566): 3*(p1*(face+ - face-) + p2*(edges+ - edges-) + p3*(corners+ -
+// corners-)) along axis A, directions visited in l order.
+// ---------------------------------------------------------------------------
+template <int A, typename Get>
+__device__ __forceinline__ float grad_axis(const Get &get) {
+  float fp = 0.f, fm = 0.f, ep = 0.f, em = 0.f, cp = 0.f, cmn = 0.f;
+#pragma unroll
+  for (int l = 1; l < 27; l++) {
+    const int ca = A == 0 ? kCX[l] : (A == 1 ? kCY[l] : kCZ[l]);
+    if (ca == 0) continue;
+    const int nnz = (kCX[l] != 0) + (kCY[l] != 0) + (kCZ[l] != 0);
+    const float val = get(kCX[l], kCY[l], kCZ[l]);
+    if (nnz == 1) {
+      if (ca > 0) fp = val; else fm = val;
+    } else if (nnz == 2) {
+      if (ca > 0) ep += val; else em += val;
+    } else {
+      if (ca > 0) cp += val; else cmn += val;
+    }
+  }
+  return 3.f * (wgt(1) * (fp - fm) + wgt(7) * (ep - em) + wgt(19) * (cp - cmn));
+}
+
+// ---------------------------------------------------------------------------
+// Source evaluation for k_fused: loads the moments of one node and returns its
+// shifted post-collision populations (and phase populations).
+// ---------------------------------------------------------------------------
+template <int NC, bool ADD>
+__device__ __forceinline__ void eval_source(const CFields &src, const float *__restrict__ n0a,
+                                            const float *__restrict__ n1a,
+                                            const float *__restrict__ n2a, const Params &p,
+                                            int gx, int gy, int gz, float *h, float *gq) {
+  const int plane = p.nx * p.ny;
+  const int id = gx + p.nx * gy + plane * gz;
+  float m[10];
+#pragma unroll
+  for (int q = 0; q < 10; q++) m[q] = src.f[q][id];
+  if constexpr (NC == 2) {
+    const float phi = src.f[TSLB_PHI][id];
+    const float n0 = n0a[id], n1 = n1a[id], n2 = n2a[id];
+    // CSF curvature shift, recursive_2c_TSLB3D.f90:614-636
+    float curv = 0.f;
+    const float ind = phi * (1.f - phi) * (n0 * n0 + n1 * n1 + n2 * n2);
+    if (ind > 0.2f) {
+      auto nat = [&](const float *__restrict__ arr, int dx, int dy, int dz) -> float {
+        int xx, yy, zz;
+        if (!src_coord(gx + dx, p.nx, p.px, xx) || !src_coord(gy + dy, p.ny, p.py, yy) ||
+            !src_coord(gz + dz, p.nz, p.pz, zz))
+          return 0.f;
+        return arr[xx + p.nx * yy + plane * zz];
+      };
+      const float dxx = grad_axis<0>([&](int a, int b, int c) { return nat(n0a, a, b, c); });
+      const float dyy = grad_axis<1>([&](int a, int b, int c) { return nat(n1a, a, b, c); });
+      const float dzz = grad_axis<2>([&](int a, int b, int c) { return nat(n2a, a, b, c); });
+      curv = -(dxx + dyy + dzz);
+    }
+    const float sx = p.sigma * curv * n0, sy = p.sigma * curv * n1, sz = p.sigma * curv * n2;
+    const float esx = sx + p.fx + p.lin_a * m[1];
+    const float esy = sy + p.fy + p.lin_a * m[2];
+    const float esz = sz + p.fz + p.lin_a * m[3];
+    const NodeC nc = node_consts(m, esx, esy, esz, p, true, false);
+    all_pops<true, false>(nc, h);
+    // D3Q7 phase field with sharpening, recursive_2c_TSLB3D.f90:808-831
+    const float p1g = 0.125f;
+    const float sh = p1g * p.sharp_c * phi * (1.f - phi);
+    gq[0] = 0.25f * phi;
+    const float base = p1g * phi;
+    gq[1] = base * (1.f + 3.f * m[1]) + sh * n0;
+    gq[2] = base * (1.f - 3.f * m[1]) - sh * n0;
+    gq[3] = base * (1.f + 3.f * m[2]) + sh * n1;
+    gq[4] = base * (1.f - 3.f * m[2]) - sh * n1;
+    gq[5] = base * (1.f + 3.f * m[3]) + sh * n2;
+    gq[6] = base * (1.f - 3.f * m[3]) - sh * n2;
+  } else {
+    if (p.shift) {
+      const NodeC nc = node_consts(m, p.fx + p.lin_a * m[1], p.fy + p.lin_a * m[2],
+                                   p.fz + p.lin_a * m[3], p, true, false);
+      all_pops<true, false>(nc, h);
+    } else {
+      const NodeC nc = node_consts(m, 0.f, 0.f, 0.f, p, false, ADD);
+      all_pops<false, ADD>(nc, h);
+    }
+  }
+}
+
+// moment sums of shifted populations for one destination plane
+struct Acc {
+  float s0, sx, sy, sz, sxx, syy, szz, sxy, sxz, syz, phi;
+};
+__device__ __forceinline__ void acc_zero(Acc &a) {
+  a.s0 = a.sx = a.sy = a.sz = a.sxx = a.syy = a.szz = a.sxy = a.sxz = a.syz = a.phi = 0.f;
+}
+
+// accumulate the 9 populations of one c_z layer, p[cy+1][cx+1]
+template <int CZ>
+__device__ __forceinline__ void acc_layer(Acc &a, const float (&p)[3][3]) {
+  float rs[3], rd[3], rx[3];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    rx[r] = p[r][2] + p[r][0];
+    rs[r] = rx[r] + p[r][1];
+    rd[r] = p[r][2] - p[r][0];
+  }
+  const float U0 = rs[0] + rs[1] + rs[2];
+  const float U1 = rs[2] - rs[0];
+  const float U2 = rs[2] + rs[0];
+  const float X1 = rd[0] + rd[1] + rd[2];
+  const float X1Y = rd[2] - rd[0];
+  const float X2 = rx[0] + rx[1] + rx[2];
+  a.s0 += U0;
+  a.sy += U1;
+  a.syy += U2;
+  a.sx += X1;
+  a.sxy += X1Y;
+  a.sxx += X2;
+  if constexpr (CZ != 0) {
+    a.sz += cm<CZ>(U0);
+    a.szz += U0;
+    a.sxz += cm<CZ>(X1);
+    a.syz += cm<CZ>(U1);
+  }
+}
+
+// gather the c_z = CZ layer for destination column (tx,ty) from LDS
+template <int CZ, bool MASKED>
+__device__ __forceinline__ void gather_layer(Acc &a, const float (*lds)[NSLOT], int tx, int ty,
+                                             float own, uint32_t bm) {
+  float p[3][3];
+#pragma unroll
+  for (int cy = -1; cy <= 1; cy++)
+#pragma unroll
+    for (int cx = -1; cx <= 1; cx++) {
+      const int l = dir_of(cx, cy, CZ);
+      float v;
+      if (cx == 0 && cy == 0) v = own;
+      else v = lds[qslot(l)][(ty + 1 - cy) * SW + (tx + 1 - cx)];
+      if (MASKED && ((bm >> l) & 1u)) v = 0.f;
+      p[cy + 1][cx + 1] = v;
+    }
+  acc_layer<CZ>(a, p);
+}
+
+// bounce-back replacements for plane of the own source: the population a
+// destination sent towards an invalid source comes back, f_l(y) = h_opp(l)(y).
+template <int CZ>
+__device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int tx, int ty,
+                                         float h0, float h5, float h6, uint32_t bm) {
+  float p[3][3];
+  const int own = (ty + 1) * SW + (tx + 1);
+#pragma unroll
+  for (int cy = -1; cy <= 1; cy++)
+#pragma unroll
+    for (int cx = -1; cx <= 1; cx++) {
+      const int l = dir_of(cx, cy, CZ);
+      const int o = kOPP[l];
+      float v = 0.f;
+      if ((bm >> l) & 1u) {
+        if (o == 0) v = h0;
+        else if (o == 5) v = h5;
+        else if (o == 6) v = h6;
+        else v = lds[qslot(o)][own];
+      }
+      p[cy + 1][cx + 1] = v;
+    }
+  acc_layer<CZ>(a, p);
+}
+
+template <int NC, bool MASK, bool ADD>
+__global__ __launch_bounds__(NTHR) void k_fused(CFields src, Fields dst, const float *__restrict__ n0a,
+                                                const float *__restrict__ n1a,
+                                                const float *__restrict__ n2a,
+                                                const uint32_t *__restrict__ links, Params p) {
+  constexpr int NQ = NC == 2 ? 28 : 24;
+  __shared__ float lds[NQ][NSLOT];
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  int b = blockIdx.x;
+  const int tile_x = b % p.tiles_x;
+  b /= p.tiles_x;
+  const int tile_y = b % p.tiles_y;
+  const int zc = b / p.tiles_y;
+  const int x0 = tile_x * TX, y0 = tile_y * TY;
+  const int z0 = zc * p.z_chunk;
+  const int z1 = min(z0 + p.z_chunk, p.nz);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool active = x < p.nx && y < p.ny;
+  const int plane = p.nx * p.ny;
+
+  // own source column
+  int ox = 0, oy = 0;
+  const bool oxy = src_coord(x, p.nx, p.px, ox) && src_coord(y, p.ny, p.py, oy);
+  // ring source slot of this thread
+  int rsx = 0, rsy = 0;
+  if (tid < NRING) {
+    if (tid < SW) { rsx = tid; rsy = 0; }
+    else if (tid < 2 * SW) { rsx = tid - SW; rsy = SH - 1; }
+    else if (tid < 2 * SW + TY) { rsx = 0; rsy = 1 + (tid - 2 * SW); }
+    else { rsx = SW - 1; rsy = 1 + (tid - 2 * SW - TY); }
+  }
+  int rx = 0, ry = 0;
+  const bool rxy = tid < NRING && src_coord(x0 + rsx - 1, p.nx, p.px, rx) &&
+                   src_coord(y0 + rsy - 1, p.ny, p.py, ry);
+
+  // ghost-wall bounce-back bits of this column
+  uint32_t wall_xy = 0;
+  if (!p.px) {
+    if (x == 0) wall_xy |= dirmask(0, 1);
+    if (x == p.nx - 1) wall_xy |= dirmask(0, -1);
+  }
+  if (!p.py) {
+    if (y == 0) wall_xy |= dirmask(1, 1);
+    if (y == p.ny - 1) wall_xy |= dirmask(1, -1);
+  }
+  auto plane_mask = [&](int z) -> uint32_t {
+    uint32_t m = wall_xy;
+    if (!p.pz) {
+      if (z == 0) m |= dirmask(2, 1);
+      if (z == p.nz - 1) m |= dirmask(2, -1);
+    }
+    if constexpr (MASK) {
+      if (active && z >= 0 && z < p.nz) m |= links[x + p.nx * y + plane * z];
+    }
+    return m;
+  };
+
+  Acc aA, aB, aC;
+  acc_zero(aA);
+  acc_zero(aB);
+  uint32_t bmA = 0, bmB = plane_mask(z0 - 1);
+
+  for (int zs = z0 - 1; zs <= z1; zs++) {
+    int gz = 0;
+    const bool zok = src_coord(zs, p.nz, p.pz, gz);
+    float h[27], gq[7];
+    float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
+    // ---- own source slot
+    {
+      const int own = (ty + 1) * SW + (tx + 1);
+      bool ok = zok && oxy;
+      if constexpr (MASK) {
+        if (ok) ok = (links[ox + p.nx * oy + plane * gz] & kSolid) == 0u;
+      }
+      if (ok) {
+        eval_source<NC, ADD>(src, n0a, n1a, n2a, p, ox, oy, gz, h, gq);
+#pragma unroll
+        for (int l = 1; l < 27; l++)
+          if (l != 5 && l != 6) lds[qslot(l)][own] = h[l];
+        h0 = h[0];
+        h5 = h[5];
+        h6 = h[6];
+        if constexpr (NC == 2) {
+#pragma unroll
+          for (int l = 1; l <= 4; l++) lds[24 + l - 1][own] = gq[l];
+          g0 = gq[0];
+          g5 = gq[5];
+          g6 = gq[6];
+        }
+      }
+    }
+    // ---- halo ring slot
+    if (tid < NRING) {
+      const int slot = rsy * SW + rsx;
+      bool ok = zok && rxy;
+      if constexpr (MASK) {
+        if (ok) ok = (links[rx + p.nx * ry + plane * gz] & kSolid) == 0u;
+      }
+      if (ok) {
+        eval_source<NC, ADD>(src, n0a, n1a, n2a, p, rx, ry, gz, h, gq);
+#pragma unroll
+        for (int l = 1; l < 27; l++)
+          if (l != 5 && l != 6) lds[qslot(l)][slot] = h[l];
+        if constexpr (NC == 2) {
+#pragma unroll
+          for (int l = 1; l <= 4; l++) lds[24 + l - 1][slot] = gq[l];
+        }
+      }
+    }
+    __syncthreads();
+
+    const uint32_t bmC = plane_mask(zs + 1);
+    acc_zero(aC);
+    if (active) {
+      if ((bmA | bmB | bmC) == 0u) {
+        gather_layer<-1, false>(aA, lds, tx, ty, h6, 0u);
+        gather_layer<0, false>(aB, lds, tx, ty, h0, 0u);
+        gather_layer<1, false>(aC, lds, tx, ty, h5, 0u);
+        if constexpr (NC == 2) {
+          aA.phi += g6;
+          aB.phi += g0 + lds[24][(ty + 1) * SW + tx] + lds[25][(ty + 1) * SW + tx + 2] +
+                    lds[26][ty * SW + tx + 1] + lds[27][(ty + 2) * SW + tx + 1];
+          aC.phi += g5;
+        }
+      } else {
+        gather_layer<-1, true>(aA, lds, tx, ty, h6, bmA);
+        gather_layer<0, true>(aB, lds, tx, ty, h0, bmB);
+        gather_layer<1, true>(aC, lds, tx, ty, h5, bmC);
+        bb_layer<-1>(aB, lds, tx, ty, h0, h5, h6, bmB);
+        bb_layer<0>(aB, lds, tx, ty, h0, h5, h6, bmB);
+        bb_layer<1>(aB, lds, tx, ty, h0, h5, h6, bmB);
+        if constexpr (NC == 2) {
+          const int own = (ty + 1) * SW + tx + 1;
+          const float gown[7] = {g0, lds[24][own], lds[25][own], lds[26][own], lds[27][own], g5, g6};
+          // phase populations: l=6 into plane zs-1, l=0..4 into zs, l=5 into zs+1
+          if (!((bmA >> 6) & 1u)) aA.phi += g6;
+          float s = g0;
+          if (!((bmB >> 1) & 1u)) s += lds[24][(ty + 1) * SW + tx];
+          if (!((bmB >> 2) & 1u)) s += lds[25][(ty + 1) * SW + tx + 2];
+          if (!((bmB >> 3) & 1u)) s += lds[26][ty * SW + tx + 1];
+          if (!((bmB >> 4) & 1u)) s += lds[27][(ty + 2) * SW + tx + 1];
+#pragma unroll
+          for (int l = 1; l <= 6; l++)
+            if ((bmB >> l) & 1u) s += gown[kOPP[l]];
+          aB.phi += s;
+          if (!((bmC >> 5) & 1u)) aC.phi += g5;
+        }
+      }
+    }
+
+    // ---- finalize destination plane zs-1
+    const int zd = zs - 1;
+    if (active && zd >= z0 && zd < z1) {
+      const int id = x + p.nx * y + plane * zd;
+      if (MASK && (bmA & kSolid)) {
+#pragma unroll
+        for (int q = 0; q < TSLB_NFIELDS; q++)
+          if (q < 10 || NC == 2) dst.f[q][id] = src.f[q][id];
+      } else {
+        const float rho = 1.f + aA.s0;
+        const float inv = 1.f / rho;
+        const float u = aA.sx * inv, v = aA.sy * inv, w = aA.sz * inv;
+        const float t = aA.s0 * (1.f / 3.f);
+        dst.f[0][id] = aA.s0;
+        dst.f[1][id] = u;
+        dst.f[2][id] = v;
+        dst.f[3][id] = w;
+        dst.f[4][id] = aA.sxx - t - rho * u * u;
+        dst.f[5][id] = aA.syy - t - rho * v * v;
+        dst.f[6][id] = aA.szz - t - rho * w * w;
+        dst.f[7][id] = aA.sxy - rho * u * v;
+        dst.f[8][id] = aA.sxz - rho * u * w;
+        dst.f[9][id] = aA.syz - rho * v * w;
+        if constexpr (NC == 2) dst.f[10][id] = aA.phi;
+      }
+    }
+    __syncthreads();
+    aA = aB;
+    aB = aC;
+    bmA = bmB;
+    bmB = bmC;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2: normals n = grad(phi)/(|grad(phi)| + 1e-9) (recursive_2c_TSLB3D.f90:
+// 555-571); phi read across a non-periodic face at the nearest interior node;
+// zero on solid nodes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_normals(const float *__restrict__ phi,
+                                                 float *__restrict__ n0,
+                                                 float *__restrict__ n1,
+                                                 float *__restrict__ n2,
+                                                 const uint32_t *__restrict__ links, int has_mask,
+                                                 Params p) {
+  const int n = p.nx * p.ny * p.nz;
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= n) return;
+  if (has_mask && (links[id] & kSolid)) {
+    n0[id] = n1[id] = n2[id] = 0.f;
+    return;
+  }
+  const int x = id % p.nx;
+  const int y = (id / p.nx) % p.ny;
+  const int z = id / (p.nx * p.ny);
+  auto ph = [&](int dx, int dy, int dz) -> float {
+    int xx = x + dx, yy = y + dy, zz = z + dz;
+    if (xx < 0 || xx >= p.nx) xx = p.px ? wrapc(xx, p.nx) : (xx < 0 ? 0 : p.nx - 1);
+    if (yy < 0 || yy >= p.ny) yy = p.py ? wrapc(yy, p.ny) : (yy < 0 ? 0 : p.ny - 1);
+    if (zz < 0 || zz >= p.nz) zz = p.pz ? wrapc(zz, p.nz) : (zz < 0 ? 0 : p.nz - 1);
+    return phi[xx + p.nx * yy + p.nx * p.ny * zz];
+  };
+  const float gx = grad_axis<0>(ph), gy = grad_axis<1>(ph), gz = grad_axis<2>(ph);
+  const float mod = sqrtf(gx * gx + gy * gy + gz * gz);
+  const float den = mod + 1e-9f;
+  n0[id] = gx / den;
+  n1[id] = gy / den;
+  n2[id] = gz / den;
+}
+
+// ---------------------------------------------------------------------------
+// Initial phi: one advection by the unsharpened D3Q7 equilibrium
+// (recursive_2c_TSLB3D.f90:256-283 writes g(i+1,..,1) etc.), bounce-back at
+// invalid sources.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prepush(const float *__restrict__ phi,
+                                                 const float *__restrict__ u,
+                                                 const float *__restrict__ v,
+                                                 const float *__restrict__ w,
+                                                 float *__restrict__ out,
+                                                 const uint32_t *__restrict__ links, int has_mask,
+                                                 Params p) {
+  const int n = p.nx * p.ny * p.nz;
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= n) return;
+  if (has_mask && (links[id] & kSolid)) {
+    out[id] = phi[id];
+    return;
+  }
+  const int x = id % p.nx;
+  const int y = (id / p.nx) % p.ny;
+  const int z = id / (p.nx * p.ny);
+  const float *vel[3] = {u, v, w};
+  float own[7];
+  own[0] = 0.25f * phi[id];
+  for (int a = 0; a < 3; a++) {
+    own[1 + 2 * a] = 0.125f * phi[id] * (1.f + 3.f * vel[a][id]);
+    own[2 + 2 * a] = 0.125f * phi[id] * (1.f - 3.f * vel[a][id]);
+  }
+  float s = own[0];
+  for (int l = 1; l <= 6; l++) {
+    int sx, sy, sz;
+    bool ok = src_coord(x - kCX[l], p.nx, p.px, sx) && src_coord(y - kCY[l], p.ny, p.py, sy) &&
+              src_coord(z - kCZ[l], p.nz, p.pz, sz);
+    int sid = ok ? sx + p.nx * sy + p.nx * p.ny * sz : 0;
+    if (ok && has_mask && (links[sid] & kSolid)) ok = false;
+    if (!ok) {
+      s += own[kOPP[l]];
+    } else {
+      const int a = (l - 1) / 2;
+      const float sgn = (l & 1) ? 1.f : -1.f;
+      s += 0.125f * phi[sid] * (1.f + sgn * 3.f * vel[a][sid]);
+    }
+  }
+  out[id] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Link masks from isfluid: bit l set when the source y - c_l of fluid node y
+// is a solid node; solid nodes carry kSolid and all direction bits.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_links(const int32_t *__restrict__ isf,
+                                               uint32_t *__restrict__ links, Params p) {
+  const int n = p.nx * p.ny * p.nz;
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= n) return;
+  if (isf[id] == 0) {
+    links[id] = kSolid | kAllDirs;
+    return;
+  }
+  const int x = id % p.nx;
+  const int y = (id / p.nx) % p.ny;
+  const int z = id / (p.nx * p.ny);
+  uint32_t m = 0;
+  for (int l = 1; l < 27; l++) {
+    int sx, sy, sz;
+    if (src_coord(x - kCX[l], p.nx, p.px, sx) && src_coord(y - kCY[l], p.ny, p.py, sy) &&
+        src_coord(z - kCZ[l], p.nz, p.pz, sz)) {
+      if (isf[sx + p.nx * sy + p.nx * p.ny * sz] == 0) m |= 1u << l;
+    }
+  }
+  links[id] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Moment-space z-plane BC (TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292):
+// plane 0 <- plane 1 with u = v = 0 and w = w_in; plane nz-1 <- plane nz-2.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_zplane_bc(Fields f, const float *__restrict__ w_in,
+                                                   const float *__restrict__ phi_in, int ncomp,
+                                                   Params p) {
+  const int plane = p.nx * p.ny;
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= plane) return;
+  const int lo = id, lo1 = id + plane;
+  const int hi = id + plane * (p.nz - 1), hi1 = id + plane * (p.nz - 2);
+  const int nf = ncomp == 2 ? TSLB_NFIELDS : 10;
+  for (int q = 0; q < nf; q++) f.f[q][hi] = f.f[q][hi1];
+  for (int q = 0; q < nf; q++) {
+    if (q == TSLB_U || q == TSLB_V) f.f[q][lo] = 0.f;
+    else if (q == TSLB_W) f.f[q][lo] = w_in[id];
+    else if (q == TSLB_PHI && phi_in) f.f[q][lo] = phi_in[id];
+    else f.f[q][lo] = f.f[q][lo1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG on the device (TSLB_newBCS/vars_single_3D_module.f90:
+// 102-165); integer-exact, identical to the host tslb_rand_noseeded.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline float rand_noseeded_impl(int i, int j, int k, int l) {
+  int isub = i % 178 + 1, jsub = j % 178 + 1, ksub = k % 178 + 1;
+  int lsub = l % 169, msub;
+  const float cc = 362436.0f / 16777216.0f;
+  const float cd = 7654321.0f / 16777216.0f;
+  const float cmod = 16777213.0f / 16777216.0f;
+  float u[2];
+  for (int pass = 0; pass < 2; pass++) {
+    float s = 0.0f, t = 0.5f;
+    for (int jj = 0; jj < 24; jj++) {
+      msub = ((isub * jsub) % 179 * ksub) % 179;
+      isub = jsub;
+      jsub = ksub;
+      ksub = msub;
+      lsub = (53 * lsub + 1) % 169;
+      if ((lsub * msub) % 64 >= 32) s = s + t;
+      t = 0.5f * t;
+    }
+    u[pass] = s;
+  }
+  float uni = u[0] - u[1];
+  if (uni < 0.0f) uni = uni + 1.0f;
+  float csub = cc - cd;
+  if (csub < 0.0f) csub = csub + cmod;
+  uni = uni - csub;
+  if (uni < 0.0f) uni = uni + 1.0f;
+  return uni;
+}
+
+// field += amp * (2 r - 1), r = rand_noseeded(i, j, k, l) with 1-based i, j, k
+// (seeding as TSLB_2C-mpi/lb_kernels_module.f90:970-977)
+__global__ __launch_bounds__(256) void k_seed_noise(float *__restrict__ a, float amp, int l, Params p) {
+  const int n = p.nx * p.ny * p.nz;
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= n) return;
+  const int x = id % p.nx;
+  const int y = (id / p.nx) % p.ny;
+  const int z = id / (p.nx * p.ny);
+  const float r = rand_noseeded_impl(x + 1, y + 1, z + 1, l);
+  a[id] = a[id] + amp * (2.0f * r - 1.0f);
+}
+
+__global__ __launch_bounds__(256) void k_fill(float *__restrict__ a, float v, int n) {
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id < n) a[id] = v;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics: per-block partial sums, then one block sums the partials.
+// ---------------------------------------------------------------------------
+constexpr int NRED = 7;
+__global__ __launch_bounds__(256) void k_reduce_part(CFields f, const uint32_t *__restrict__ links,
+                                                     int has_mask, Params p,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[NRED][256];
+  const int n = p.nx * p.ny * p.nz;
+  double acc[NRED] = {0, 0, 0, 0, 0, 0, 0};
+  for (int id = blockIdx.x * 256 + threadIdx.x; id < n; id += gridDim.x * 256) {
+    if (has_mask && (links[id] & kSolid)) continue;
+    const int x = id % p.nx;
+    const int y = (id / p.nx) % p.ny;
+    const int z = id / (p.nx * p.ny);
+    const double rho = 1.0 + (double)f.f[0][id];
+    const double u = f.f[1][id], v = f.f[2][id], w = f.f[3][id];
+    acc[0] += rho;
+    if (p.ncomp == 2) acc[1] += (double)f.f[TSLB_PHI][id];
+    acc[2] += 0.5 * rho * (u * u + v * v + w * w);
+    acc[3] += rho * u;
+    acc[4] += rho * v;
+    acc[5] += rho * w;
+    auto at = [&](int q, int dx, int dy, int dz) -> double {
+      int xx = x + dx, yy = y + dy, zz = z + dz;
+      if (xx < 0 || xx >= p.nx) xx = p.px ? wrapc(xx, p.nx) : (xx < 0 ? 0 : p.nx - 1);
+      if (yy < 0 || yy >= p.ny) yy = p.py ? wrapc(yy, p.ny) : (yy < 0 ? 0 : p.ny - 1);
+      if (zz < 0 || zz >= p.nz) zz = p.pz ? wrapc(zz, p.nz) : (zz < 0 ? 0 : p.nz - 1);
+      return (double)f.f[q][xx + p.nx * yy + p.nx * p.ny * zz];
+    };
+    const double wx = 0.5 * (at(3, 0, 1, 0) - at(3, 0, -1, 0)) - 0.5 * (at(2, 0, 0, 1) - at(2, 0, 0, -1));
+    const double wy = 0.5 * (at(1, 0, 0, 1) - at(1, 0, 0, -1)) - 0.5 * (at(3, 1, 0, 0) - at(3, -1, 0, 0));
+    const double wz = 0.5 * (at(2, 1, 0, 0) - at(2, -1, 0, 0)) - 0.5 * (at(1, 0, 1, 0) - at(1, 0, -1, 0));
+    acc[6] += 0.5 * (wx * wx + wy * wy + wz * wz);
+  }
+  for (int q = 0; q < NRED; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < NRED; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NRED; q++) part[(size_t)blockIdx.x * NRED + q] = sh[q][0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__ part, int nblk,
+                                                      double *__restrict__ out) {
+  __shared__ double sh[NRED][256];
+  double acc[NRED] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int q = 0; q < NRED; q++) acc[q] += part[(size_t)b * NRED + q];
+  for (int q = 0; q < NRED; q++) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < NRED; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NRED; q++) out[q] = sh[q][0];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCK(expr)                                                                 \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(TSLB_E_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kRedBlocks = 1024;
+
+}  // namespace
+
+struct tslb_ctx {
+  tslb_params prm;
+  Params kp;
+  hipStream_t stream = nullptr;
+  size_t n = 0, fs = 0;  // nodes, field stride (elements)
+  float *buf[2] = {nullptr, nullptr};
+  int cur = 0;
+  float *nrm = nullptr;  // 3 * fs
+  uint32_t *links = nullptr;
+  int has_mask = 0;
+  int normals_fresh = 0;
+  double *part = nullptr, *red = nullptr;
+  float *w_in = nullptr, *phi_in = nullptr;
+  int zbc = 0, zbc_phi = 0;
+  size_t bytes = 0;
+  int timing = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[2];
+  double ms[2] = {0, 0};
+  int64_t launches[2] = {0, 0};
+};
+
+namespace {
+
+Fields fields_of(float *base, size_t fs) {
+  Fields f;
+  for (int q = 0; q < TSLB_NFIELDS; q++) f.f[q] = base + (size_t)q * fs;
+  return f;
+}
+CFields cfields_of(const float *base, size_t fs) {
+  CFields f;
+  for (int q = 0; q < TSLB_NFIELDS; q++) f.f[q] = base + (size_t)q * fs;
+  return f;
+}
+
+int dev_alloc(tslb_ctx *c, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(TSLB_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                  hipGetErrorString(e));
+  }
+  c->bytes += bytes;
+  return TSLB_OK;
+}
+
+hipEvent_t take_event(tslb_ctx *c) {
+  hipEvent_t e = nullptr;
+  if (!c->ev_pool.empty()) {
+    e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+  } else {
+    hipEventCreate(&e);
+  }
+  return e;
+}
+
+int drain_timing(tslb_ctx *c) {
+  for (int k = 0; k < 2; k++) {
+    for (auto &pr : c->pending[k]) {
+      HIPCK(hipEventSynchronize(pr.second));
+      float ms = 0.f;
+      HIPCK(hipEventElapsedTime(&ms, pr.first, pr.second));
+      c->ms[k] += ms;
+      c->launches[k] += 1;
+      c->ev_pool.push_back(pr.first);
+      c->ev_pool.push_back(pr.second);
+    }
+    c->pending[k].clear();
+  }
+  return TSLB_OK;
+}
+
+int launch_normals(tslb_ctx *c) {
+  const float *base = c->buf[c->cur];
+  const int nb = (int)((c->n + 255) / 256);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    e0 = take_event(c);
+    e1 = take_event(c);
+    HIPCK(hipEventRecord(e0, c->stream));
+  }
+  hipLaunchKernelGGL(k_normals, dim3(nb), dim3(256), 0, c->stream, base + TSLB_PHI * c->fs,
+                     c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->links, c->has_mask, c->kp);
+  HIPCK(hipGetLastError());
+  if (c->timing) {
+    HIPCK(hipEventRecord(e1, c->stream));
+    c->pending[1].push_back({e0, e1});
+  }
+  c->normals_fresh = 1;
+  return TSLB_OK;
+}
+
+int launch_fused(tslb_ctx *c) {
+  const float *src = c->buf[c->cur];
+  float *dst = c->buf[1 - c->cur];
+  const Params &p = c->kp;
+  const int nblk = p.tiles_x * p.tiles_y * p.nzc;
+  CFields s = cfields_of(src, c->fs);
+  Fields d = fields_of(dst, c->fs);
+  const float *n0 = c->nrm, *n1 = c->nrm + c->fs, *n2 = c->nrm + 2 * c->fs;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    e0 = take_event(c);
+    e1 = take_event(c);
+    HIPCK(hipEventRecord(e0, c->stream));
+  }
+  const bool add = p.ncomp == 1 && !p.shift &&
+                   (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
+  if (p.ncomp == 2) {
+    if (c->has_mask)
+      hipLaunchKernelGGL((k_fused<2, true, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+    else
+      hipLaunchKernelGGL((k_fused<2, false, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+  } else if (add) {
+    if (c->has_mask)
+      hipLaunchKernelGGL((k_fused<1, true, true>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+    else
+      hipLaunchKernelGGL((k_fused<1, false, true>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+  } else {
+    if (c->has_mask)
+      hipLaunchKernelGGL((k_fused<1, true, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+    else
+      hipLaunchKernelGGL((k_fused<1, false, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+  }
+  HIPCK(hipGetLastError());
+  if (c->timing) {
+    HIPCK(hipEventRecord(e1, c->stream));
+    c->pending[0].push_back({e0, e1});
+    if (c->pending[0].size() >= 512) {
+      int rc = drain_timing(c);
+      if (rc) return rc;
+    }
+  }
+  if (p.ncomp == 1) {
+    // phi is carried unchanged by one-component runs
+    HIPCK(hipMemcpyAsync(dst + TSLB_PHI * c->fs, src + TSLB_PHI * c->fs, c->n * sizeof(float),
+                         hipMemcpyDeviceToDevice, c->stream));
+  }
+  c->cur = 1 - c->cur;
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int launch_zbc(tslb_ctx *c) {
+  if (!c->zbc) return TSLB_OK;
+  const int plane = c->prm.nx * c->prm.ny;
+  hipLaunchKernelGGL(k_zplane_bc, dim3((plane + 255) / 256), dim3(256), 0, c->stream,
+                     fields_of(c->buf[c->cur], c->fs), c->w_in, c->zbc_phi ? c->phi_in : nullptr,
+                     c->prm.ncomp, c->kp);
+  HIPCK(hipGetLastError());
+  return TSLB_OK;
+}
+
+int check_ctx(tslb_ctx *c) {
+  if (!c) return fail(TSLB_E_ARG, "null context");
+  HIPCK(hipSetDevice(c->prm.device));
+  return TSLB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void tslb_default_params(tslb_params *p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->nx = p->ny = p->nz = 16;
+  p->ncomp = 1;
+  p->periodic[0] = p->periodic[1] = p->periodic[2] = 1;
+  p->force_mode = TSLB_FORCE_ADDITIVE;
+  p->storage = TSLB_STORAGE_FP32;
+  p->device = 0;
+  p->tau = 0.8f;
+}
+
+int tslb_create(tslb_ctx **out, const tslb_params *p) {
+  if (!out || !p) return fail(TSLB_E_ARG, "null argument");
+  *out = nullptr;
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1)
+    return fail(TSLB_E_ARG, "lattice extents must be >= 1");
+  const long long n = (long long)p->nx * p->ny * p->nz;
+  if (n > 0x7fffffffLL) return fail(TSLB_E_ARG, "nx*ny*nz must fit in int32");
+  if (p->ncomp != 1 && p->ncomp != 2) return fail(TSLB_E_ARG, "ncomp must be 1 or 2");
+  if (!(p->tau > 0.5f)) return fail(TSLB_E_ARG, "tau must be > 0.5");
+  if (p->storage != TSLB_STORAGE_FP32)
+    return fail(TSLB_E_ARG, "only FP32 storage is available in this build");
+  if (p->force_mode != TSLB_FORCE_ADDITIVE && p->force_mode != TSLB_FORCE_SHIFT)
+    return fail(TSLB_E_ARG, "force_mode must be TSLB_FORCE_ADDITIVE or TSLB_FORCE_SHIFT");
+  int ndev = 0;
+  HIPCK(hipGetDeviceCount(&ndev));
+  if (p->device < 0 || p->device >= ndev) return fail(TSLB_E_ARG, "device ordinal out of range");
+  HIPCK(hipSetDevice(p->device));
+
+  tslb_ctx *c = new tslb_ctx();
+  c->prm = *p;
+  c->n = (size_t)n;
+  c->fs = (c->n + 63) & ~(size_t)63;
+  Params &k = c->kp;
+  k.nx = p->nx;
+  k.ny = p->ny;
+  k.nz = p->nz;
+  k.px = p->periodic[0] ? 1 : 0;
+  k.py = p->periodic[1] ? 1 : 0;
+  k.pz = p->periodic[2] ? 1 : 0;
+  k.ncomp = p->ncomp;
+  k.shift = (p->ncomp == 2 || p->force_mode == TSLB_FORCE_SHIFT) ? 1 : 0;
+  k.om1 = 1.f - 1.f / p->tau;
+  k.sigma = p->sigma;
+  k.sharp_c = p->sharp_c;
+  k.fx = p->force[0];
+  k.fy = p->force[1];
+  k.fz = p->force[2];
+  k.lin_a = p->lin_force;
+  k.z_chunk = p->nz < 32 ? p->nz : 32;
+  k.tiles_x = (p->nx + TX - 1) / TX;
+  k.tiles_y = (p->ny + TY - 1) / TY;
+  k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
+
+  int rc = TSLB_OK;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(TSLB_E_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+  }
+  const size_t fb = c->fs * TSLB_NFIELDS * sizeof(float);
+  if ((rc = dev_alloc(c, (void **)&c->buf[0], fb)) || (rc = dev_alloc(c, (void **)&c->buf[1], fb)) ||
+      (p->ncomp == 2 && (rc = dev_alloc(c, (void **)&c->nrm, 3 * c->fs * sizeof(float)))) ||
+      (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
+      (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
+    std::string msg = g_err;
+    tslb_destroy(c);
+    return fail(rc, msg);
+  }
+  // zero state: rho-1 = 0, all moments 0
+  for (int b = 0; b < 2; b++) {
+    const int tot = (int)(c->fs * TSLB_NFIELDS);
+    hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->buf[b], 0.f, tot);
+  }
+  if (c->nrm) {
+    const int tot = (int)(3 * c->fs);
+    hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->nrm, 0.f, tot);
+  }
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    tslb_destroy(c);
+    return fail(TSLB_E_HIP, std::string("initial fill failed: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return TSLB_OK;
+}
+
+int tslb_destroy(tslb_ctx *c) {
+  if (!c) return TSLB_OK;
+  hipSetDevice(c->prm.device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 2; k++) {
+    for (auto &pr : c->pending[k]) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+  }
+  for (auto e : c->ev_pool) hipEventDestroy(e);
+  hipFree(c->buf[0]);
+  hipFree(c->buf[1]);
+  hipFree(c->nrm);
+  hipFree(c->links);
+  hipFree(c->part);
+  hipFree(c->red);
+  hipFree(c->w_in);
+  hipFree(c->phi_in);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return TSLB_OK;
+}
+
+int tslb_set_isfluid(tslb_ctx *c, const int32_t *isf) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!isf) {
+    c->has_mask = 0;
+    return TSLB_OK;
+  }
+  if (!c->links) {
+    if ((rc = dev_alloc(c, (void **)&c->links, c->n * sizeof(uint32_t)))) return rc;
+  }
+  int32_t *tmp = nullptr;
+  HIPCK(hipMalloc((void **)&tmp, c->n * sizeof(int32_t)));
+  hipError_t e = hipMemcpyAsync(tmp, isf, c->n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_links, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, tmp,
+                       c->links, c->kp);
+    e = hipStreamSynchronize(c->stream);
+  }
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(TSLB_E_HIP, std::string("set_isfluid: ") + hipGetErrorString(e));
+  c->has_mask = 1;
+  return TSLB_OK;
+}
+
+int tslb_upload_field(tslb_ctx *c, int field, const float *data) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (field < 0 || field >= TSLB_NFIELDS || !data) return fail(TSLB_E_ARG, "bad field or pointer");
+  float *d = c->buf[c->cur] + (size_t)field * c->fs;
+  if (field == TSLB_RHO) {
+    std::vector<float> tmp(c->n);
+    for (size_t i = 0; i < c->n; i++) tmp[i] = data[i] - 1.0f;
+    HIPCK(hipMemcpyAsync(d, tmp.data(), c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  } else {
+    HIPCK(hipMemcpyAsync(d, data, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+  }
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int tslb_upload_moments(tslb_ctx *c, const float *m) {
+  if (!m) return fail(TSLB_E_ARG, "null pointer");
+  for (int q = 0; q < TSLB_NFIELDS; q++) {
+    int rc = tslb_upload_field(c, q, m + (size_t)q * (c ? c->n : 0));
+    if (rc) return rc;
+  }
+  return TSLB_OK;
+}
+
+int tslb_download_field(tslb_ctx *c, int field, float *data) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (field < 0 || field >= TSLB_NFIELDS || !data) return fail(TSLB_E_ARG, "bad field or pointer");
+  const float *d = c->buf[c->cur] + (size_t)field * c->fs;
+  HIPCK(hipMemcpyAsync(data, d, c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (field == TSLB_RHO)
+    for (size_t i = 0; i < c->n; i++) data[i] = 1.0f + data[i];
+  return TSLB_OK;
+}
+
+int tslb_download_moments(tslb_ctx *c, float *m) {
+  if (!m) return fail(TSLB_E_ARG, "null pointer");
+  for (int q = 0; q < TSLB_NFIELDS; q++) {
+    int rc = tslb_download_field(c, q, m + (size_t)q * (c ? c->n : 0));
+    if (rc) return rc;
+  }
+  return TSLB_OK;
+}
+
+int tslb_download_normals(tslb_ctx *c, float *n3) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!n3) return fail(TSLB_E_ARG, "null pointer");
+  if (c->prm.ncomp != 2) return fail(TSLB_E_STATE, "normals exist only for two-component runs");
+  if (!c->normals_fresh && (rc = launch_normals(c))) return rc;
+  for (int a = 0; a < 3; a++)
+    HIPCK(hipMemcpyAsync(n3 + (size_t)a * c->n, c->nrm + (size_t)a * c->fs, c->n * sizeof(float),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return TSLB_OK;
+}
+
+int tslb_init_equilibrium(tslb_ctx *c, int prepush) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  float *base = c->buf[c->cur];
+  const int nb = (int)((c->n + 255) / 256);
+  for (int q = TSLB_PXX; q <= TSLB_PYZ; q++)
+    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, c->stream, base + q * c->fs, 0.f, (int)c->n);
+  if (c->prm.ncomp == 2 && prepush) {
+    float *tmp = c->buf[1 - c->cur] + TSLB_PHI * c->fs;
+    hipLaunchKernelGGL(k_prepush, dim3(nb), dim3(256), 0, c->stream, base + TSLB_PHI * c->fs,
+                       base + TSLB_U * c->fs, base + TSLB_V * c->fs, base + TSLB_W * c->fs, tmp,
+                       c->links, c->has_mask, c->kp);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(base + TSLB_PHI * c->fs, tmp, c->n * sizeof(float), hipMemcpyDeviceToDevice,
+                         c->stream));
+  }
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int tslb_normals(tslb_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->prm.ncomp != 2) return TSLB_OK;
+  return launch_normals(c);
+}
+
+int tslb_collide_stream(tslb_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->prm.ncomp == 2 && !c->normals_fresh) return fail(TSLB_E_STATE, "tslb_normals() must precede tslb_collide_stream()");
+  return launch_fused(c);
+}
+
+int tslb_step(tslb_ctx *c, int nsteps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(TSLB_E_ARG, "nsteps < 0");
+  for (int s = 0; s < nsteps; s++) {
+    if (c->prm.ncomp == 2 && (rc = launch_normals(c))) return rc;
+    if ((rc = launch_fused(c))) return rc;
+    if ((rc = launch_zbc(c))) return rc;
+  }
+  return TSLB_OK;
+}
+
+int tslb_set_zplane_bc(tslb_ctx *c, const float *w_in, const float *phi_in) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!w_in) {
+    c->zbc = 0;
+    return TSLB_OK;
+  }
+  if (c->prm.nz < 3 || c->prm.periodic[2]) return fail(TSLB_E_ARG, "z-plane BC needs nz >= 3 and a non-periodic z axis");
+  const size_t plane = (size_t)c->prm.nx * c->prm.ny;
+  if (!c->w_in && (rc = dev_alloc(c, (void **)&c->w_in, plane * sizeof(float)))) return rc;
+  if (!c->phi_in && (rc = dev_alloc(c, (void **)&c->phi_in, plane * sizeof(float)))) return rc;
+  HIPCK(hipMemcpyAsync(c->w_in, w_in, plane * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  c->zbc_phi = 0;
+  if (phi_in && c->prm.ncomp == 2) {
+    HIPCK(hipMemcpyAsync(c->phi_in, phi_in, plane * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    c->zbc_phi = 1;
+  }
+  c->zbc = 1;
+  if ((rc = launch_zbc(c))) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int tslb_seed_noise(tslb_ctx *c, int field, float amp, int l) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (field < 0 || field >= TSLB_NFIELDS) return fail(TSLB_E_ARG, "bad field");
+  float *d = c->buf[c->cur] + (size_t)field * c->fs;
+  hipLaunchKernelGGL(k_seed_noise, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, d, amp, l, c->kp);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int tslb_reduce(tslb_ctx *c, double out[7]) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) return fail(TSLB_E_ARG, "null pointer");
+  const int nblk = (int)((c->n + 255) / 256) < kRedBlocks ? (int)((c->n + 255) / 256) : kRedBlocks;
+  hipLaunchKernelGGL(k_reduce_part, dim3(nblk), dim3(256), 0, c->stream, cfields_of(c->buf[c->cur], c->fs),
+                     c->links, c->has_mask, c->kp, c->part);
+  HIPCK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, c->part, nblk, c->red);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(out, c->red, NRED * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  return TSLB_OK;
+}
+
+int tslb_sync(tslb_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  return TSLB_OK;
+}
+
+int tslb_timing_enable(tslb_ctx *c, int on) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = drain_timing(c))) return rc;
+  c->timing = on ? 1 : 0;
+  c->ms[0] = c->ms[1] = 0.0;
+  c->launches[0] = c->launches[1] = 0;
+  return TSLB_OK;
+}
+
+int tslb_timing_read(tslb_ctx *c, int k, double *total_ms, int64_t *launches) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (k < 0 || k > 1 || !total_ms || !launches) return fail(TSLB_E_ARG, "bad timing query");
+  if ((rc = drain_timing(c))) return rc;
+  *total_ms = c->ms[k];
+  *launches = c->launches[k];
+  return TSLB_OK;
+}
+
+int tslb_mem_info(size_t *free_b, size_t *total_b) {
+  if (!free_b || !total_b) return fail(TSLB_E_ARG, "null pointer");
+  HIPCK(hipMemGetInfo(free_b, total_b));
+  return TSLB_OK;
+}
+
+int tslb_host_mem_mb(double *peak_rss_mb, double *total_ram_mb) {
+  if (!peak_rss_mb || !total_ram_mb) return fail(TSLB_E_ARG, "null pointer");
+  struct rusage ru;
+  if (getrusage(RUSAGE_SELF, &ru) != 0) return fail(TSLB_E_STATE, "getrusage failed");
+  *peak_rss_mb = (double)ru.ru_maxrss / 1024.0;
+  struct sysinfo si;
+  if (sysinfo(&si) != 0) return fail(TSLB_E_STATE, "sysinfo failed");
+  *total_ram_mb = (double)si.totalram * (double)si.mem_unit / (1024.0 * 1024.0);
+  return TSLB_OK;
+}
+
+float tslb_rand_noseeded(int i, int j, int k, int l) { return rand_noseeded_impl(i, j, k, l); }
+
+float tslb_gauss_noseeded(int i, int j, int k, int l) {
+  float d1 = tslb_rand_noseeded(i, j, k, l);
+  const int kk = (int)lroundf(d1 * 50.0f);
+  const float d2 = tslb_rand_noseeded(i, j, k, l + kk);
+  d1 = d1 * (1.0f - 1e-30f) + 1e-30f;
+  return sqrtf(-2.0f * logf(d1)) * cosf(2.0f * 3.14159265358979323846f * d2);
+}
+
+int tslb_device_bytes(tslb_ctx *c, size_t *bytes) {
+  if (!c || !bytes) return fail(TSLB_E_ARG, "null argument");
+  *bytes = c->bytes;
+  return TSLB_OK;
+}
+
+int tslb_last_error(char *buf, int len) {
+  if (!buf || len <= 0) return TSLB_E_ARG;
+  snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return TSLB_OK;
+}
+
+const char *tslb_version(void) { return "tslb 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+/*
+ * tslb.h -- C ABI of the MI355X TSLB library (libtslb.so).
+ *
+ * The thread-safe recursive-regularized D3Q27 lattice Boltzmann step (TSLB) for
+ * one- and two-component flow on one GPU.  The state lives on the device as
+ * structure-of-arrays moment fields (rho, u, v, w, Pi^neq (6), phi); one step is
+ * a fused moment-space collide + pull-stream.  Host arrays are Fortran-ordered
+ * (1:nx,1:ny,1:nz), i fastest, and stay owned by the caller; nothing is
+ * retained after a call returns.
+ *
+ * Conventions: every call returns 0 on success and a negative TSLB_E* code on
+ * failure; tslb_last_error() gives the message of the last failure on the
+ * calling thread.  A context is not re-entrant: one host thread drives it.
+ * Work is queued on the context's own HIP stream (default priority);
+ * tslb_sync() waits for it.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).  In the reference, these are no-argument
+ * Fortran subroutines on module globals, inside an `!$acc data` region that
+ * owns device residency (recursive_2c_TSLB3D.f90:316-317,841).
+ */
+#ifndef TSLB_H
+#define TSLB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSLB_OK 0
+#define TSLB_E_ARG (-1)      /* invalid argument */
+#define TSLB_E_HIP (-2)      /* HIP runtime error */
+#define TSLB_E_NOMEM (-3)    /* device allocation failed */
+#define TSLB_E_STATE (-4)    /* call not valid in the current state */
+
+#define TSLB_FORCE_ADDITIVE 0 /* f_l += w_l c_l.F / cs^2  (recursive_TSLB3D.f90:571-707) */
+#define TSLB_FORCE_SHIFT 1    /* u_eq = u + F            (lb_kernels_module.f90:585-587)  */
+
+#define TSLB_STORAGE_FP32 0
+#define TSLB_STORAGE_BF16 1
+#define TSLB_STORAGE_FP16 2
+
+/* Moment field indices (the order of tslb_upload_moments' packed buffer). */
+#define TSLB_RHO 0
+#define TSLB_U 1
+#define TSLB_V 2
+#define TSLB_W 3
+#define TSLB_PXX 4
+#define TSLB_PYY 5
+#define TSLB_PZZ 6
+#define TSLB_PXY 7
+#define TSLB_PXZ 8
+#define TSLB_PYZ 9
+#define TSLB_PHI 10
+#define TSLB_NFIELDS 11
+
+/* Run parameters.  Replaces the hard-coded parameter block of the drivers
+ * (recursive_2c_TSLB3D.f90:20-52, recursive_TSLB3D.f90:20-48). */
+typedef struct tslb_params {
+  int32_t nx, ny, nz;    /* lattice extent (physical nodes, no ghosts)            */
+  int32_t ncomp;         /* 1: single component; 2: colour-gradient two-component  */
+  int32_t periodic[3];   /* 1: periodic axis; 0: halfway bounce-back wall just
+                            outside both faces of that axis                         */
+  int32_t force_mode;    /* TSLB_FORCE_*; two-component runs always shift          */
+  int32_t storage;       /* TSLB_STORAGE_*: device storage of the moment fields     */
+  int32_t device;        /* HIP device ordinal                                      */
+  float tau;             /* relaxation time, omega = 1/tau                          */
+  float sigma;           /* surface tension (CSF), recursive_2c_TSLB3D.f90:27        */
+  float sharp_c;         /* phase-field sharpening coefficient, :28                  */
+  float force[3];        /* constant body force fx, fy, fz                           */
+  float lin_force;       /* linear forcing coefficient A: u_eq += A u (shift mode)   */
+} tslb_params;
+
+typedef struct tslb_ctx tslb_ctx;
+
+/* Fill *p with defaults (periodic 16^3, one component, tau 0.8, FP32). */
+void tslb_default_params(tslb_params *p);
+
+/* Allocate the device state (two moment buffers, normals, scratch).
+ * Replaces the allocate block and the `!$acc data copy(...)` region:
+ * recursive_2c_TSLB3D.f90:54-59,316-317. */
+int tslb_create(tslb_ctx **out, const tslb_params *p);
+
+/* Free the device state.  Replaces `!$acc end data` (recursive_2c_TSLB3D.f90:841). */
+int tslb_destroy(tslb_ctx *ctx);
+
+/* Geometry: isfluid (nx,ny,nz) int32, 1 fluid / 0 solid.  Solid nodes keep
+ * their moments; fluid neighbours see halfway bounce-back.  Replaces the
+ * geometry block recursive_2c_TSLB3D.f90:79-85 and bcs_all_bback_2c
+ * (boundary_cds_3D_module.f90:10-75).  NULL clears the mask (all fluid). */
+int tslb_set_isfluid(tslb_ctx *ctx, const int32_t *isfluid);
+
+/* Host -> device moments.  `m` holds TSLB_NFIELDS arrays of nx*ny*nz floats,
+ * packed in TSLB_* order.  Replaces `!$acc update device` of the moment arrays
+ * (recursive_2c_TSLB3D.f90:316-317). */
+int tslb_upload_moments(tslb_ctx *ctx, const float *m);
+/* One field (TSLB_*), nx*ny*nz floats. */
+int tslb_upload_field(tslb_ctx *ctx, int field, const float *data);
+
+/* Device -> host moments (same layout); used at output steps.  Replaces
+ * `!$acc update host(rho,phi,u,v,w)` (recursive_2c_TSLB3D.f90:586,597). */
+int tslb_download_moments(tslb_ctx *ctx, float *m);
+int tslb_download_field(tslb_ctx *ctx, int field, float *data);
+
+/* Colour-gradient normals n = grad(phi)/(|grad(phi)|+1e-9) of the current
+ * state, 3*nx*ny*nz floats (nx, ny, nz components).  Valid after a step or
+ * after tslb_normals().  Replaces `!$acc update host(normx,...)`. */
+int tslb_download_normals(tslb_ctx *ctx, float *n3);
+
+/* Initial state from the uploaded rho, u, v, w, phi: Pi^neq = 0 (f = feq at
+ * every fluid node) and, when prepush_phi != 0, phi advected once by the
+ * unsharpened D3Q7 equilibrium, as the reference writes g already pushed.
+ * Replaces recursive_2c_TSLB3D.f90:106-287 and initial_conditions
+ * (TSLB_2C-mpi/lb_kernels_module.f90:938-1142). */
+int tslb_init_equilibrium(tslb_ctx *ctx, int prepush_phi);
+
+/* Advance nsteps full steps: (two-component) colour-gradient normals, then the
+ * fused moment-space collide + pull-stream with periodic wrap, halfway
+ * bounce-back and forcing, then the moment-space plane BCs if set.  Replaces
+ * one pass of the time loop recursive_2c_TSLB3D.f90:344-840 (K1 moments,
+ * K2 normals, K3 fused collide+push, bcs_2c_turbulent_jet / pbcs). */
+int tslb_step(tslb_ctx *ctx, int nsteps);
+
+/* Split entry points, for bisection against the reference's kernel order. */
+/* K2 colour gradient (recursive_2c_TSLB3D.f90:543-577). */
+int tslb_normals(tslb_ctx *ctx);
+/* K3 fused collide + stream followed by K1 moments of the streamed
+ * populations, as one moment-space pass (recursive_2c_TSLB3D.f90:607-836 then
+ * :346-542 of the next step); uses the normals of the last tslb_normals(). */
+int tslb_collide_stream(tslb_ctx *ctx);
+
+/* Moment-space z-plane boundary conditions applied after every step:
+ * plane z=1 gets rho, Pi from z=2, u=v=0, w=w_in(x,y); plane z=nz is a
+ * zero-gradient copy of z=nz-1.  Replaces bcs_TSLB_only_z_turbojet
+ * (TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292).  w_in is nx*ny floats
+ * (NULL disables the BC); for a two-component run phi_in sets phi on z=1
+ * (NULL keeps the copy from z=2). */
+int tslb_set_zplane_bc(tslb_ctx *ctx, const float *w_in, const float *phi_in);
+
+/* Seeded noise: field += amp * (2 r - 1) with r = tslb_rand_noseeded(i, j, k, l)
+ * at 1-based node coordinates, computed on the device.  Replaces the seeded
+ * initial perturbation of initial_conditions
+ * (TSLB_2C-mpi/lb_kernels_module.f90:970-977).  Field TSLB_RHO perturbs rho. */
+int tslb_seed_noise(tslb_ctx *ctx, int field, float amp, int l);
+
+/* FP64 diagnostics over fluid nodes: out[0] sum rho, [1] sum phi,
+ * [2] sum rho|u|^2/2, [3..5] sum rho u, [6] enstrophy sum |curl u|^2/2
+ * (central differences).  Synchronous. */
+int tslb_reduce(tslb_ctx *ctx, double out[7]);
+
+/* Wait for all queued work of the context. */
+int tslb_sync(tslb_ctx *ctx);
+
+/* Kernel timing with HIP events on the context stream: when enabled every
+ * fused collide-stream launch is bracketed by events.  tslb_timing_read()
+ * synchronises and returns the total milliseconds and the launch count of the
+ * fused kernel (k=0) and the normals kernel (k=1) since the last reset. */
+int tslb_timing_enable(tslb_ctx *ctx, int on);
+int tslb_timing_read(tslb_ctx *ctx, int k, double *total_ms, int64_t *launches);
+
+/* Device memory of the current device, bytes (hipMemGetInfo).  Replaces
+ * get_memory_gpu (print_module.f90:546-592). */
+int tslb_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* Host memory: peak resident set (MB) and total RAM (MB).  Replaces get_mem /
+ * get_totalram (TSLB_newBCS/get_mem.c:23-39, get_ram.c:49-66). */
+int tslb_host_mem_mb(double *peak_rss_mb, double *total_ram_mb);
+
+/* Counter-based RNG, REAL(4) result in [0,1): TSLB_newBCS/vars_single_3D_module.f90:102-165. */
+float tslb_rand_noseeded(int i, int j, int k, int l);
+/* Box-Muller Gaussian from two draws: vars_single_3D_module.f90:78-100. */
+float tslb_gauss_noseeded(int i, int j, int k, int l);
+
+/* Bytes of device memory the context holds. */
+int tslb_device_bytes(tslb_ctx *ctx, size_t *bytes);
+
+/* Message of the last failure on this thread (always NUL-terminated). */
+int tslb_last_error(char *buf, int len);
+
+/* Library version string, e.g. "tslb 0.1 gfx950". */
+const char *tslb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSLB_H */
