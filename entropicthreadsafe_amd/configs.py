@@ -59,7 +59,7 @@ def rand_noseeded_grid(nx, ny, nz, l):
 class Case:
     name: str
     shape: Tuple[int, int, int]            # (nz, ny, nx)
-    lattice: Dict                           # Lattice(...) keyword arguments
+    lattice: Dict                           # Lattice(nx, ny, nz, **lattice) keyword arguments
     fields: Dict[str, np.ndarray]           # rho, u, v, w, phi
     noise: List[Tuple[str, float, int]] = field(default_factory=list)
     isfluid: Optional[np.ndarray] = None
@@ -105,7 +105,7 @@ def tgv(n=64, u0=0.05, re=1600.0, length=None):
     u = u0 * np.sin(kk * i) * np.cos(kk * j) * np.cos(kk * k)
     v = -u0 * np.cos(kk * i) * np.sin(kk * j) * np.cos(kk * k)
     z = np.zeros_like(u)
-    return Case("tgv", (n, n, n), dict(nx=n, ny=n, nz=n, ncomp=1, tau=tau, periodic=(1, 1, 1)),
+    return Case("tgv", (n, n, n), dict(ncomp=1, tau=tau, periodic=(1, 1, 1)),
                 dict(rho=np.ones_like(u), u=u, v=v, w=z, phi=z.copy()),
                 notes=f"nu={nu:.6g} tau={tau:.6g}")
 
@@ -123,8 +123,7 @@ def droplet(n=256, radius=40.0, sigma=0.02, beta=0.7, noise=1e-3, tau=1.0):
     phi = (r2 <= radius ** 2).astype(np.float64)
     z = np.zeros_like(phi)
     return Case("droplet", (n, n, n),
-                dict(nx=n, ny=n, nz=n, ncomp=2, tau=tau, sigma=sigma, sharp_c=3.0 * beta,
-                     periodic=(1, 1, 1)),
+                dict(ncomp=2, tau=tau, sigma=sigma, sharp_c=3.0 * beta, periodic=(1, 1, 1)),
                 dict(rho=np.ones_like(phi), u=z, v=z.copy(), w=z.copy(), phi=phi),
                 noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)])
 
@@ -148,8 +147,7 @@ def channel(nx=512, ny=256, nz=256, re_tau=180.0, u_tau=0.005, pert=0.05, ub_ove
     z = np.zeros_like(u)
     amp = pert * ub
     return Case("channel", (nz, ny, nx),
-                dict(nx=nx, ny=ny, nz=nz, ncomp=1, tau=tau, force=(fx, 0.0, 0.0),
-                     force_mode=0, periodic=(1, 0, 1)),
+                dict(ncomp=1, tau=tau, force=(fx, 0.0, 0.0), force_mode=0, periodic=(1, 0, 1)),
                 dict(rho=np.ones_like(u), u=u, v=z, w=z.copy(), phi=z.copy()),
                 noise=[("u", amp, 101), ("v", amp, 102), ("w", amp, 103)],
                 notes=f"nu={nu:.6g} tau={tau:.6g} Fx={fx:.6g}")
@@ -208,8 +206,8 @@ def emulsion(n=512, ndrop=200, rmin=8.0, rmax=16.0, sigma=0.01, beta=0.7, tau=0.
     z = np.zeros((n, n, n), dtype=np.float32)
     vol = float(phi.sum()) / n ** 3
     return Case("emulsion", (n, n, n),
-                dict(nx=n, ny=n, nz=n, ncomp=2, tau=tau, sigma=sigma, sharp_c=3.0 * beta,
-                     lin_force=lin_force, periodic=(1, 1, 1)),
+                dict(ncomp=2, tau=tau, sigma=sigma, sharp_c=3.0 * beta, lin_force=lin_force,
+                     periodic=(1, 1, 1)),
                 dict(rho=np.ones_like(z), u=z, v=z, w=z, phi=phi),
                 noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)],
                 notes=f"{len(drops)} droplets, volume fraction {vol:.4f}")

@@ -4,12 +4,18 @@ Same seeded inputs on both sides; the oracle runs the reference's
 population-push algorithm.  Tolerances (per-field max relative error, see
 tests/parity.py):
   * GPU FP32 vs FP64 oracle: <= 1e-5 on every field;
-  * GPU FP32 vs FP32 oracle: <= 1e-5 on rho, u, v, w, phi (the FP32 oracle's
-    own Pi^neq carries ~1e-4 round-off from summing unshifted populations, so
-    Pi is judged against FP64 only).
-Two-component cases can flip the curvature gate (phi(1-phi)|n|^2 > 0.2,
-recursive_2c_TSLB3D.f90:615-616) at round-off; their tolerance is stated in
-the test.
+  * GPU FP32 vs FP32 oracle: <= 1e-5 + e_ref on every field, where e_ref is
+    the FP32 oracle's own error against the FP64 oracle (the reference's FP32
+    arithmetic sums unshifted populations and drifts by up to ~1e-4 on Pi^neq
+    and ~1e-5 on u near tau = 1/2; the GPU path sums f - w and stays closer to
+    FP64, so the two FP32 results may differ by the reference's own error).
+Two-component runs pass the curvature gate phi(1-phi)|n|^2 > 0.2
+(recursive_2c_TSLB3D.f90:615-616): a 6e-8 relative perturbation of the input
+flips it at some interface node after ~20 steps, and the flip moves u by ~1e-4
+of the velocity scale even between two FP64 runs of the oracle.  Two-component
+pointwise parity is therefore checked over 15 steps; longer runs are compared
+through flip-insensitive quantities (mass, phase mass, Laplace pressure jump,
+kinetic energy).
 """
 import numpy as np
 import pytest
@@ -54,19 +60,21 @@ def run_oracle(kw, fields, steps, isfluid=None, prepush=True, dtype=np.float64):
     return m
 
 
-def check(kw, fields, steps, isfluid=None, tol64=1e-5, tol32=1e-5, fields32=("rho", "u", "v", "w", "phi")):
+def check(kw, fields, steps, isfluid=None, tol64=1e-5, tol32=1e-5):
     got = run_gpu(kw, fields, steps, isfluid)
     ref64 = run_oracle(kw, fields, steps, isfluid)
     ref32 = run_oracle(kw, fields, steps, isfluid, dtype=np.float32)
     e64 = field_errors(got, ref64)
     e32 = field_errors(got, ref32)
-    print("vs FP64", {k: f"{v:.2e}" for k, v in e64.items()})
-    print("vs FP32", {k: f"{v:.2e}" for k, v in e32.items()})
+    eref = field_errors(ref32, ref64)
+    print("GPU vs FP64   ", {k: f"{v:.2e}" for k, v in e64.items()})
+    print("GPU vs FP32   ", {k: f"{v:.2e}" for k, v in e32.items()})
+    print("FP32 vs FP64  ", {k: f"{v:.2e}" for k, v in eref.items()})
     assert np.isfinite(got).all()
     for f, v in e64.items():
         assert v <= tol64, ("fp64", f, v)
-    for f in fields32:
-        assert e32[f] <= tol32, ("fp32", f, e32[f])
+    for f, v in e32.items():
+        assert v <= tol32 + eref[f], ("fp32", f, v, eref[f])
     return got, ref64
 
 
@@ -104,7 +112,45 @@ def test_obstacle_mask_shift_force_closed_box():
 
 def test_droplet_periodic_2c():
     case = configs.droplet(n=32, radius=8.0, sigma=0.02, beta=0.7, noise=1e-3)
-    got, ref = check(case.lattice, case.host_fields(), 60, tol64=1e-5)
+    check(case.lattice, case.host_fields(), 15)
+
+
+def test_droplet_long_run_integrals():
+    # 400 steps; pressure jump and kinetic energy averaged over steps 200..400
+    n, R, sigma = 32, 8.0, 0.02
+    case = configs.droplet(n=n, radius=R, sigma=sigma, beta=0.7, noise=1e-3)
+    f = case.host_fields()
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    r = np.sqrt((i + 1 - n / 2) ** 2 + (j + 1 - n / 2) ** 2 + (k + 1 - n / 2) ** 2)
+
+    def integrals(m):
+        m = m.astype(np.float64)
+        dp = (m[0][r < R - 3].mean() - m[0][r > R + 5].mean()) / 3.0
+        ke = 0.5 * (m[0] * (m[1] ** 2 + m[2] ** 2 + m[3] ** 2)).sum()
+        return np.array([m[0].sum(), m[10].sum(), dp, ke])
+
+    samples = {"gpu": [], "oracle": []}
+    with Lattice(n, n, n, **case.lattice) as lat:
+        for name in ("rho", "u", "v", "w", "phi"):
+            lat.upload_field(name, f[name])
+        lat.init_equilibrium(True)
+        lat.step(200)
+        for _ in range(10):
+            lat.step(20)
+            samples["gpu"].append(integrals(lat.download()))
+    o = po.Oracle(n, n, n, **_oracle_kwargs(case.lattice))
+    o.init(*[f[q].astype(np.float64) for q in ("rho", "u", "v", "w", "phi")], True)
+    o.step(200)
+    for _ in range(10):
+        o.step(20)
+        samples["oracle"].append(integrals(o.moments()))
+    g = np.array(samples["gpu"])
+    h = np.array(samples["oracle"])
+    print("gpu mean", g.mean(0), "oracle mean", h.mean(0), "2 sigma/R", 2 * sigma / R)
+    assert np.all(np.abs(g[:, 0] - h[:, 0]) < 1e-6 * h[:, 0])
+    assert np.all(np.abs(g[:, 1] - h[:, 1]) < 1e-6 * h[:, 1])
+    assert abs(g[:, 2].mean() - h[:, 2].mean()) < 0.05 * abs(h[:, 2].mean())
+    assert abs(g[:, 3].mean() - h[:, 3].mean()) < 0.2 * h[:, 3].mean()
 
 
 def test_two_component_walls_mask_forcing():
@@ -119,7 +165,7 @@ def test_two_component_walls_mask_forcing():
     isf[15:18, 2:5, 3:6] = 0
     kw = dict(ncomp=2, tau=0.9, sigma=0.02, sharp_c=2.1, force=(1e-5, 0, 0), lin_force=1e-4,
               periodic=(1, 0, 1))
-    check(kw, f, 40, isfluid=isf)
+    check(kw, f, 15, isfluid=isf)
 
 
 def test_split_entry_points_equal_step():
