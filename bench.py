@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="lattice edge (512 for the named workload)")
+    ap.add_argument("--workload", default="emulsion", choices=("emulsion", "channel", "tgv"),
+                    help="emulsion (2C, the named workload), channel (1C, n x n/2 x n/2), tgv (1C, n^3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -95,12 +97,21 @@ def main():
 
     n = args.n
     t0 = time.time()
-    case = configs.emulsion(n=n)
-    log(f"bench: emulsion {n}^3 set up in {time.time() - t0:.1f} s ({case.notes})")
-    nodes = n ** 3
-    with Lattice(n, n, n, device=0, **case.lattice) as lat:
+    if args.workload == "emulsion":
+        case = configs.emulsion(n=n)
+    elif args.workload == "channel":
+        case = configs.channel(nx=n, ny=n // 2, nz=n // 2)
+    else:
+        case = configs.tgv(n=n)
+    nz, ny, nx = case.shape
+    ncomp = case.lattice["ncomp"]
+    log(f"bench: {args.workload} {nx}x{ny}x{nz} set up in {time.time() - t0:.1f} s ({case.notes})")
+    nodes = nx * ny * nz
+    with Lattice(nx, ny, nz, device=0, **case.lattice) as lat:
         case.apply(lat)
         log(f"bench: state on device ({lat.device_bytes() / 2**30:.2f} GiB)")
+        copy_gbs = lat.stream_copy_gbs()
+        log(f"bench: plain copy kernel {copy_gbs:.0f} GB/s")
         lat.step(args.warmup)
         lat.sync()
         r0 = lat.reduce()
@@ -119,7 +130,8 @@ def main():
         f"phi drift {(r1[1] - r0[1]) / max(r0[1], 1e-30):.2e}")
 
     mlups = nodes * args.steps / wall / 1e6
-    bytes_per_launch = 8.0 * FIELDS_2C * nodes          # read + write of 11 FP32 fields
+    nfields = 11 if ncomp == 2 else 10
+    bytes_per_launch = 8.0 * nfields * nodes          # read + write of the FP32 moment fields
     t_fused = fused_ms / max(fused_n, 1) / 1e3
     achieved = bytes_per_launch / t_fused / 1e9
     t_step = (fused_ms + norm_ms) / max(fused_n, 1) / 1e3
@@ -137,17 +149,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded droplets and velocity noise)",
-        "config": {"workload": f"emulsion-2c-{n}^3", "nx": n, "ny": n, "nz": n, "components": 2,
-                   "storage": "fp32", "droplets": case.notes, "tau": case.lattice["tau"],
-                   "sigma": case.lattice["sigma"], "sharp_c": case.lattice["sharp_c"],
-                   "lin_force": case.lattice["lin_force"]},
+        "config": {"workload": f"{args.workload}-{ncomp}c-{nx}x{ny}x{nz}", "nx": nx, "ny": ny, "nz": nz,
+                   "components": ncomp, "storage": "fp32", "notes": case.notes,
+                   **{k: v for k, v in case.lattice.items() if k in ("tau", "sigma", "sharp_c", "lin_force", "force")}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_fused<2>", "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "kernel": f"k_fused<{ncomp}>", "algorithmic_bytes_per_launch": bytes_per_launch,
                      "mean_launch_ms": round(t_fused * 1e3, 4)},
         "step_roofline_frac": round(bytes_per_launch / t_step / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_stream_copy_gbs": round(copy_gbs, 1),
+        "fused_frac_of_copy": round(achieved / copy_gbs, 4),
     }
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "emulsion":
         try:
             out["cpu_baseline"] = cpu_baseline(n, case, args.cpu_budget)
         except Exception as e:  # the baseline is reported, never the product

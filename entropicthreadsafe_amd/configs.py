@@ -156,17 +156,20 @@ def channel(nx=512, ny=256, nz=256, re_tau=180.0, u_tau=0.005, pert=0.05, ub_ove
 def place_droplets(n, ndrop, rmin, rmax, seed_l=7):
     """Non-overlapping droplets by seeded rejection sampling (periodic box).
 
-    Candidate m uses rand_noseeded(m, 1..4, 1, seed_l) for centre and radius.
+    Candidate m draws centre and radius from rand_noseeded(a, b, q, seed_l)
+    with a = m mod 178 + 1, b = m div 178 + 1, q = 1..4 (the generator reduces
+    each seed modulo 178, so m itself cannot be the only varying seed).
     """
     from . import tslb as _t  # the generator of the library (host C)
     drops = []
     m = 0
-    while len(drops) < ndrop and m < 200000:
+    while len(drops) < ndrop and m < 178 * 178:
+        a, b = m % 178 + 1, m // 178 + 1
         m += 1
-        cx = _t.rand_noseeded(m, 1, 1, seed_l) * n
-        cy = _t.rand_noseeded(m, 2, 1, seed_l) * n
-        cz = _t.rand_noseeded(m, 3, 1, seed_l) * n
-        r = rmin + (rmax - rmin) * _t.rand_noseeded(m, 4, 1, seed_l)
+        cx = _t.rand_noseeded(a, b, 1, seed_l) * n
+        cy = _t.rand_noseeded(a, b, 2, seed_l) * n
+        cz = _t.rand_noseeded(a, b, 3, seed_l) * n
+        r = rmin + (rmax - rmin) * _t.rand_noseeded(a, b, 4, seed_l)
         ok = True
         for (x, y, z, rr) in drops:
             d = np.array([cx - x, cy - y, cz - z])

@@ -160,8 +160,23 @@ struct NodeC {
   float f3x, f3y, f3z;                    // 3 F (additive forcing)
 };
 
-template <int L, bool SHIFT, bool ADD>
-__device__ __forceinline__ void pair_pop(const NodeC &n, float *h) {
+// Destination of the populations of one source: directions that move in the
+// x-y plane go to the LDS slot of the source, 0, 5 and 6 stay in registers.
+struct PopSink {
+  float (*lds)[NSLOT];
+  int slot;
+  float h0, h5, h6;
+  template <int L>
+  __device__ __forceinline__ void put(float v) {
+    if constexpr (L == 0) h0 = v;
+    else if constexpr (L == 5) h5 = v;
+    else if constexpr (L == 6) h6 = v;
+    else lds[qslot(L)][slot] = v;
+  }
+};
+
+template <int L, bool SHIFT, bool ADD, typename Sink>
+__device__ __forceinline__ void pair_pop(const NodeC &n, Sink &h) {
   constexpr int X = kCX[L], Y = kCY[L], Z = kCZ[L];
   const float cus = cdot<X, Y, Z>(n.sx, n.sy, n.sz);
   const float cu = SHIFT ? cdot<X, Y, Z>(n.ux, n.uy, n.uz) : cus;
@@ -186,13 +201,13 @@ __device__ __forceinline__ void pair_pop(const NodeC &n, float *h) {
   O = fmaf(-2.f, cq, O);
   if constexpr (ADD) O = O + cdot<X, Y, Z>(n.f3x, n.f3y, n.f3z);
   constexpr float w = wgt(L);
-  h[L] = w * (E + O);
-  h[kOPP[L]] = w * (E - O);
+  h.template put<L>(w * (E + O));
+  h.template put<kOPP[L]>(w * (E - O));
 }
 
-template <bool SHIFT, bool ADD>
-__device__ __forceinline__ void all_pops(const NodeC &n, float *h) {
-  h[0] = wgt(0) * n.K0;
+template <bool SHIFT, bool ADD, typename Sink>
+__device__ __forceinline__ void all_pops(const NodeC &n, Sink &h) {
+  h.template put<0>(wgt(0) * n.K0);
   pair_pop<1, SHIFT, ADD>(n, h);
   pair_pop<3, SHIFT, ADD>(n, h);
   pair_pop<5, SHIFT, ADD>(n, h);
@@ -276,35 +291,61 @@ __device__ __forceinline__ float grad_axis(const Get &get) {
 // Source evaluation for k_fused: loads the moments of one node and returns its
 // shifted post-collision populations (and phase populations).
 // ---------------------------------------------------------------------------
-template <int NC, bool ADD>
-__device__ __forceinline__ void eval_source(const CFields &src, const float *__restrict__ n0a,
-                                            const float *__restrict__ n1a,
-                                            const float *__restrict__ n2a, const Params &p,
-                                            int gx, int gy, int gz, float *h, float *gq) {
+// kappa = -div(n) with the grouped gradient stencil on the normals; only
+// interface nodes (phi(1-phi)|n|^2 > 0.2) call it, so it is kept out of line
+// to keep the fused kernel's register footprint small.
+__device__ __noinline__ float curvature_at(const float *__restrict__ n0a,
+                                           const float *__restrict__ n1a,
+                                           const float *__restrict__ n2a, const Params &p, int gx,
+                                           int gy, int gz) {
   const int plane = p.nx * p.ny;
-  const int id = gx + p.nx * gy + plane * gz;
-  float m[10];
+  auto nat = [&](const float *__restrict__ arr, int dx, int dy, int dz) -> float {
+    int xx, yy, zz;
+    if (!src_coord(gx + dx, p.nx, p.px, xx) || !src_coord(gy + dy, p.ny, p.py, yy) ||
+        !src_coord(gz + dz, p.nz, p.pz, zz))
+      return 0.f;
+    return arr[xx + p.nx * yy + plane * zz];
+  };
+  const float dxx = grad_axis<0>([&](int a, int b, int c) { return nat(n0a, a, b, c); });
+  const float dyy = grad_axis<1>([&](int a, int b, int c) { return nat(n1a, a, b, c); });
+  const float dzz = grad_axis<2>([&](int a, int b, int c) { return nat(n2a, a, b, c); });
+  return -(dxx + dyy + dzz);
+}
+
+template <int NC>
+struct Src {
+  float m[NC == 2 ? 11 : 10];  // drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz(,phi)
+  float nn[3];                 // normals (two components)
+};
+
+template <int NC>
+__device__ __forceinline__ void load_src(const CFields &src, const float *__restrict__ n0a,
+                                         const float *__restrict__ n1a,
+                                         const float *__restrict__ n2a, int id, Src<NC> &s) {
 #pragma unroll
-  for (int q = 0; q < 10; q++) m[q] = src.f[q][id];
+  for (int q = 0; q < 10; q++) s.m[q] = src.f[q][id];
   if constexpr (NC == 2) {
-    const float phi = src.f[TSLB_PHI][id];
-    const float n0 = n0a[id], n1 = n1a[id], n2 = n2a[id];
+    s.m[10] = src.f[TSLB_PHI][id];
+    s.nn[0] = n0a[id];
+    s.nn[1] = n1a[id];
+    s.nn[2] = n2a[id];
+  }
+}
+
+template <int NC, bool ADD>
+__device__ __forceinline__ void eval_src(const Src<NC> &s, const float *__restrict__ n0a,
+                                         const float *__restrict__ n1a,
+                                         const float *__restrict__ n2a, const Params &p, int gx,
+                                         int gy, int gz, PopSink &h, float *gq) {
+  const float *m = s.m;
+  if constexpr (NC == 2) {
+    const int plane = p.nx * p.ny;
+    const float phi = m[10];
+    const float n0 = s.nn[0], n1 = s.nn[1], n2 = s.nn[2];
     // CSF curvature shift, recursive_2c_TSLB3D.f90:614-636
     float curv = 0.f;
     const float ind = phi * (1.f - phi) * (n0 * n0 + n1 * n1 + n2 * n2);
-    if (ind > 0.2f) {
-      auto nat = [&](const float *__restrict__ arr, int dx, int dy, int dz) -> float {
-        int xx, yy, zz;
-        if (!src_coord(gx + dx, p.nx, p.px, xx) || !src_coord(gy + dy, p.ny, p.py, yy) ||
-            !src_coord(gz + dz, p.nz, p.pz, zz))
-          return 0.f;
-        return arr[xx + p.nx * yy + plane * zz];
-      };
-      const float dxx = grad_axis<0>([&](int a, int b, int c) { return nat(n0a, a, b, c); });
-      const float dyy = grad_axis<1>([&](int a, int b, int c) { return nat(n1a, a, b, c); });
-      const float dzz = grad_axis<2>([&](int a, int b, int c) { return nat(n2a, a, b, c); });
-      curv = -(dxx + dyy + dzz);
-    }
+    if (ind > 0.2f) curv = curvature_at(n0a, n1a, n2a, p, gx, gy, gz);
     const float sx = p.sigma * curv * n0, sy = p.sigma * curv * n1, sz = p.sigma * curv * n2;
     const float esx = sx + p.fx + p.lin_a * m[1];
     const float esy = sy + p.fy + p.lin_a * m[2];
@@ -316,10 +357,10 @@ __device__ __forceinline__ void eval_source(const CFields &src, const float *__r
     const float sh = p1g * p.sharp_c * phi * (1.f - phi);
     gq[0] = 0.25f * phi;
     const float base = p1g * phi;
-    gq[1] = base * (1.f + 3.f * m[1]) + sh * n0;
-    gq[2] = base * (1.f - 3.f * m[1]) - sh * n0;
-    gq[3] = base * (1.f + 3.f * m[2]) + sh * n1;
-    gq[4] = base * (1.f - 3.f * m[2]) - sh * n1;
+    h.lds[24][h.slot] = base * (1.f + 3.f * m[1]) + sh * n0;
+    h.lds[25][h.slot] = base * (1.f - 3.f * m[1]) - sh * n0;
+    h.lds[26][h.slot] = base * (1.f + 3.f * m[2]) + sh * n1;
+    h.lds[27][h.slot] = base * (1.f - 3.f * m[2]) - sh * n1;
     gq[5] = base * (1.f + 3.f * m[3]) + sh * n2;
     gq[6] = base * (1.f - 3.f * m[3]) - sh * n2;
   } else {
@@ -416,8 +457,8 @@ __device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int 
   acc_layer<CZ>(a, p);
 }
 
-template <int NC, bool MASK, bool ADD>
-__global__ __launch_bounds__(NTHR) void k_fused(CFields src, Fields dst, const float *__restrict__ n0a,
+template <int NC, bool MASK, bool ADD, bool PF, int WPE>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(CFields src, Fields dst, const float *__restrict__ n0a,
                                                 const float *__restrict__ n1a,
                                                 const float *__restrict__ n2a,
                                                 const uint32_t *__restrict__ links, Params p) {
@@ -475,6 +516,27 @@ __global__ __launch_bounds__(NTHR) void k_fused(CFields src, Fields dst, const f
     return m;
   };
 
+  // source fetch of one plane: own column and (for tid < NRING) the ring slot
+  Src<NC> so, sr;
+  bool so_ok = false, sr_ok = false;
+  auto fetch = [&](int zs, Src<NC> &o, bool &ook, Src<NC> &r, bool &rok) {
+    int gz = 0;
+    const bool zok = src_coord(zs, p.nz, p.pz, gz);
+    ook = zok && oxy;
+    if (ook) {
+      const int id = ox + p.nx * oy + plane * gz;
+      if constexpr (MASK) ook = (links[id] & kSolid) == 0u;
+      if (ook) load_src<NC>(src, n0a, n1a, n2a, id, o);
+    }
+    rok = zok && rxy;
+    if (rok) {
+      const int id = rx + p.nx * ry + plane * gz;
+      if constexpr (MASK) rok = (links[id] & kSolid) == 0u;
+      if (rok) load_src<NC>(src, n0a, n1a, n2a, id, r);
+    }
+  };
+  if constexpr (PF) fetch(z0 - 1, so, so_ok, sr, sr_ok);
+
   Acc aA, aB, aC;
   acc_zero(aA);
   acc_zero(aB);
@@ -482,50 +544,37 @@ __global__ __launch_bounds__(NTHR) void k_fused(CFields src, Fields dst, const f
 
   for (int zs = z0 - 1; zs <= z1; zs++) {
     int gz = 0;
-    const bool zok = src_coord(zs, p.nz, p.pz, gz);
-    float h[27], gq[7];
+    src_coord(zs, p.nz, p.pz, gz);
+    Src<NC> co, cr;
+    bool co_ok, cr_ok;
+    if constexpr (PF) {
+      co = so;
+      cr = sr;
+      co_ok = so_ok;
+      cr_ok = sr_ok;
+      if (zs + 1 <= z1) fetch(zs + 1, so, so_ok, sr, sr_ok);
+    } else {
+      fetch(zs, co, co_ok, cr, cr_ok);
+    }
+    float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
     // ---- own source slot
-    {
-      const int own = (ty + 1) * SW + (tx + 1);
-      bool ok = zok && oxy;
-      if constexpr (MASK) {
-        if (ok) ok = (links[ox + p.nx * oy + plane * gz] & kSolid) == 0u;
-      }
-      if (ok) {
-        eval_source<NC, ADD>(src, n0a, n1a, n2a, p, ox, oy, gz, h, gq);
-#pragma unroll
-        for (int l = 1; l < 27; l++)
-          if (l != 5 && l != 6) lds[qslot(l)][own] = h[l];
-        h0 = h[0];
-        h5 = h[5];
-        h6 = h[6];
-        if constexpr (NC == 2) {
-#pragma unroll
-          for (int l = 1; l <= 4; l++) lds[24 + l - 1][own] = gq[l];
-          g0 = gq[0];
-          g5 = gq[5];
-          g6 = gq[6];
-        }
+    if (co_ok) {
+      PopSink sk{lds, (ty + 1) * SW + (tx + 1), 0.f, 0.f, 0.f};
+      eval_src<NC, ADD>(co, n0a, n1a, n2a, p, ox, oy, gz, sk, gq);
+      h0 = sk.h0;
+      h5 = sk.h5;
+      h6 = sk.h6;
+      if constexpr (NC == 2) {
+        g0 = gq[0];
+        g5 = gq[5];
+        g6 = gq[6];
       }
     }
     // ---- halo ring slot
-    if (tid < NRING) {
-      const int slot = rsy * SW + rsx;
-      bool ok = zok && rxy;
-      if constexpr (MASK) {
-        if (ok) ok = (links[rx + p.nx * ry + plane * gz] & kSolid) == 0u;
-      }
-      if (ok) {
-        eval_source<NC, ADD>(src, n0a, n1a, n2a, p, rx, ry, gz, h, gq);
-#pragma unroll
-        for (int l = 1; l < 27; l++)
-          if (l != 5 && l != 6) lds[qslot(l)][slot] = h[l];
-        if constexpr (NC == 2) {
-#pragma unroll
-          for (int l = 1; l <= 4; l++) lds[24 + l - 1][slot] = gq[l];
-        }
-      }
+    if (cr_ok) {
+      PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
+      eval_src<NC, ADD>(cr, n0a, n1a, n2a, p, rx, ry, gz, sk, gq);
     }
     __syncthreads();
 
@@ -636,6 +685,75 @@ __global__ __launch_bounds__(256) void k_normals(const float *__restrict__ phi,
   n0[id] = gx / den;
   n1[id] = gy / den;
   n2[id] = gz / den;
+}
+
+// K2, LDS-tiled: a 64x8 block marches a z-chunk, keeping three phi planes of
+// its (64+2)x(8+2) footprint in LDS; one global read of phi per node (plus the
+// footprint ring) instead of 19.
+__global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict__ phi,
+                                                         float *__restrict__ n0,
+                                                         float *__restrict__ n1,
+                                                         float *__restrict__ n2,
+                                                         const uint32_t *__restrict__ links,
+                                                         int has_mask, Params p) {
+  __shared__ float ph[3][NSLOT];
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  int b = blockIdx.x;
+  const int tile_x = b % p.tiles_x;
+  b /= p.tiles_x;
+  const int tile_y = b % p.tiles_y;
+  const int zc = b / p.tiles_y;
+  const int x0 = tile_x * TX, y0 = tile_y * TY;
+  const int z0 = zc * p.z_chunk;
+  const int z1 = min(z0 + p.z_chunk, p.nz);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool active = x < p.nx && y < p.ny;
+  const int plane = p.nx * p.ny;
+  auto clampw = [](int g, int n, int per) -> int {
+    if (g >= 0 && g < n) return g;
+    return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
+  };
+  auto load_plane = [&](int z) {
+    const int zz = clampw(z, p.nz, p.pz);
+    float *dstp = ph[((z % 3) + 3) % 3];
+    for (int sl = tid; sl < NSLOT; sl += NTHR) {
+      const int sx = sl % SW, sy = sl / SW;
+      const int gx = clampw(x0 + sx - 1, p.nx, p.px);
+      const int gy = clampw(y0 + sy - 1, p.ny, p.py);
+      dstp[sl] = phi[gx + p.nx * gy + plane * zz];
+    }
+  };
+  load_plane(z0 - 1);
+  load_plane(z0);
+  for (int z = z0; z < z1; z++) {
+    load_plane(z + 1);
+    __syncthreads();
+    if (active) {
+      const int id = x + p.nx * y + plane * z;
+      if (has_mask && (links[id] & kSolid)) {
+        n0[id] = n1[id] = n2[id] = 0.f;
+      } else {
+        auto get = [&](int dx, int dy, int dz) -> float {
+          return ph[(((z + dz) % 3) + 3) % 3][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+        };
+        const float gx = grad_axis<0>(get), gy = grad_axis<1>(get), gz = grad_axis<2>(get);
+        const float mod = sqrtf(gx * gx + gy * gy + gz * gz);
+        const float den = mod + 1e-9f;
+        n0[id] = gx / den;
+        n1[id] = gy / den;
+        n2[id] = gz / den;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// plain copy for bandwidth calibration (float4, grid-stride)
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                              long long n4) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n4; i += (long long)gridDim.x * 256)
+    b[i] = a[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -883,6 +1001,8 @@ struct tslb_ctx {
   double *part = nullptr, *red = nullptr;
   float *w_in = nullptr, *phi_in = nullptr;
   int zbc = 0, zbc_phi = 0;
+  int normals_variant = 0;  // 0: LDS-tiled, 1: one thread per node through L1
+  int variant = 1;  // bit 0: prefetch the next source plane; bit 1: cap VGPRs for 2 blocks/CU
   size_t bytes = 0;
   int timing = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -951,8 +1071,14 @@ int launch_normals(tslb_ctx *c) {
     e1 = take_event(c);
     HIPCK(hipEventRecord(e0, c->stream));
   }
-  hipLaunchKernelGGL(k_normals, dim3(nb), dim3(256), 0, c->stream, base + TSLB_PHI * c->fs,
-                     c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->links, c->has_mask, c->kp);
+  if (c->normals_variant == 0) {
+    const int nblk = c->kp.tiles_x * c->kp.tiles_y * c->kp.nzc;
+    hipLaunchKernelGGL(k_normals_tiled, dim3(nblk), dim3(NTHR), 0, c->stream, base + TSLB_PHI * c->fs,
+                       c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->links, c->has_mask, c->kp);
+  } else {
+    hipLaunchKernelGGL(k_normals, dim3(nb), dim3(256), 0, c->stream, base + TSLB_PHI * c->fs,
+                       c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->links, c->has_mask, c->kp);
+  }
   HIPCK(hipGetLastError());
   if (c->timing) {
     HIPCK(hipEventRecord(e1, c->stream));
@@ -978,22 +1104,30 @@ int launch_fused(tslb_ctx *c) {
   }
   const bool add = p.ncomp == 1 && !p.shift &&
                    (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
+  const bool pf = (c->variant & 1) != 0;
+  const bool w4 = (c->variant & 2) != 0;
+#define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                  \
+  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
+                     n0, n1, n2, c->links, p)
+#define TSLB_LAUNCH_V(NC, MASK, ADD)                 \
+  do {                                               \
+    if (pf && w4) TSLB_LAUNCH(NC, MASK, ADD, true, 4);   \
+    else if (pf) TSLB_LAUNCH(NC, MASK, ADD, true, 1);    \
+    else if (w4) TSLB_LAUNCH(NC, MASK, ADD, false, 4);   \
+    else TSLB_LAUNCH(NC, MASK, ADD, false, 1);           \
+  } while (0)
   if (p.ncomp == 2) {
-    if (c->has_mask)
-      hipLaunchKernelGGL((k_fused<2, true, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
-    else
-      hipLaunchKernelGGL((k_fused<2, false, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+    if (c->has_mask) TSLB_LAUNCH_V(2, true, false);
+    else TSLB_LAUNCH_V(2, false, false);
   } else if (add) {
-    if (c->has_mask)
-      hipLaunchKernelGGL((k_fused<1, true, true>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
-    else
-      hipLaunchKernelGGL((k_fused<1, false, true>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+    if (c->has_mask) TSLB_LAUNCH_V(1, true, true);
+    else TSLB_LAUNCH_V(1, false, true);
   } else {
-    if (c->has_mask)
-      hipLaunchKernelGGL((k_fused<1, true, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
-    else
-      hipLaunchKernelGGL((k_fused<1, false, false>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, n0, n1, n2, c->links, p);
+    if (c->has_mask) TSLB_LAUNCH_V(1, true, false);
+    else TSLB_LAUNCH_V(1, false, false);
   }
+#undef TSLB_LAUNCH_V
+#undef TSLB_LAUNCH
   HIPCK(hipGetLastError());
   if (c->timing) {
     HIPCK(hipEventRecord(e1, c->stream));
@@ -1088,6 +1222,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.tiles_y = (p->ny + TY - 1) / TY;
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
 
+  if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 3;
+  if (const char *v = getenv("TSLB_NORMALS_VARIANT")) c->normals_variant = atoi(v) & 1;
   int rc = TSLB_OK;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1380,6 +1516,30 @@ float tslb_gauss_noseeded(int i, int j, int k, int l) {
   const float d2 = tslb_rand_noseeded(i, j, k, l + kk);
   d1 = d1 * (1.0f - 1e-30f) + 1e-30f;
   return sqrtf(-2.0f * logf(d1)) * cosf(2.0f * 3.14159265358979323846f * d2);
+}
+
+int tslb_stream_copy(tslb_ctx *c, double *gbs) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!gbs) return fail(TSLB_E_ARG, "null pointer");
+  const long long n4 = (long long)(c->fs * TSLB_NFIELDS / 4);
+  hipEvent_t e0 = take_event(c), e1 = take_event(c);
+  float best = 1e30f;
+  for (int rep = 0; rep < 4; rep++) {
+    HIPCK(hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const float4 *)c->buf[c->cur],
+                       (float4 *)c->buf[1 - c->cur], n4);
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventRecord(e1, c->stream));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    if (rep > 0 && ms < best) best = ms;
+  }
+  c->ev_pool.push_back(e0);
+  c->ev_pool.push_back(e1);
+  *gbs = 2.0 * 16.0 * (double)n4 / (best * 1e-3) / 1e9;
+  return TSLB_OK;
 }
 
 int tslb_device_bytes(tslb_ctx *c, size_t *bytes) {

@@ -73,6 +73,7 @@ SIGNATURES = {
     "tslb_host_mem_mb": (ctypes.c_int, [_DP, _DP]),
     "tslb_rand_noseeded": (ctypes.c_float, [ctypes.c_int] * 4),
     "tslb_gauss_noseeded": (ctypes.c_float, [ctypes.c_int] * 4),
+    "tslb_stream_copy": (ctypes.c_int, [_P, _DP]),
     "tslb_device_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "tslb_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "tslb_version": (ctypes.c_char_p, []),
@@ -266,6 +267,12 @@ class Lattice:
         _check(self.lib.tslb_timing_read(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(cnt)),
                "tslb_timing_read")
         return ms.value, cnt.value
+
+    def stream_copy_gbs(self):
+        """Bandwidth of a plain copy kernel over the moment buffer (GB/s)."""
+        g = ctypes.c_double()
+        _check(self.lib.tslb_stream_copy(self._h, ctypes.byref(g)), "tslb_stream_copy")
+        return g.value
 
     def device_bytes(self):
         b = ctypes.c_size_t()

