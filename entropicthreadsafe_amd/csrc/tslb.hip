@@ -291,37 +291,18 @@ __device__ __forceinline__ float grad_axis(const Get &get) {
 // Source evaluation for k_fused: loads the moments of one node and returns its
 // shifted post-collision populations (and phase populations).
 // ---------------------------------------------------------------------------
-// kappa = -div(n) with the grouped gradient stencil on the normals; only
-// interface nodes (phi(1-phi)|n|^2 > 0.2) call it, so it is kept out of line
-// to keep the fused kernel's register footprint small.
-__device__ __noinline__ float curvature_at(const float *__restrict__ n0a,
-                                           const float *__restrict__ n1a,
-                                           const float *__restrict__ n2a, const Params &p, int gx,
-                                           int gy, int gz) {
-  const int plane = p.nx * p.ny;
-  auto nat = [&](const float *__restrict__ arr, int dx, int dy, int dz) -> float {
-    int xx, yy, zz;
-    if (!src_coord(gx + dx, p.nx, p.px, xx) || !src_coord(gy + dy, p.ny, p.py, yy) ||
-        !src_coord(gz + dz, p.nz, p.pz, zz))
-      return 0.f;
-    return arr[xx + p.nx * yy + plane * zz];
-  };
-  const float dxx = grad_axis<0>([&](int a, int b, int c) { return nat(n0a, a, b, c); });
-  const float dyy = grad_axis<1>([&](int a, int b, int c) { return nat(n1a, a, b, c); });
-  const float dzz = grad_axis<2>([&](int a, int b, int c) { return nat(n2a, a, b, c); });
-  return -(dxx + dyy + dzz);
-}
-
 template <int NC>
 struct Src {
   float m[NC == 2 ? 11 : 10];  // drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz(,phi)
   float nn[3];                 // normals (two components)
+  float kap;                   // gated curvature (two components)
 };
 
 template <int NC>
 __device__ __forceinline__ void load_src(const CFields &src, const float *__restrict__ n0a,
                                          const float *__restrict__ n1a,
-                                         const float *__restrict__ n2a, int id, Src<NC> &s) {
+                                         const float *__restrict__ n2a,
+                                         const float *__restrict__ ka, int id, Src<NC> &s) {
 #pragma unroll
   for (int q = 0; q < 10; q++) s.m[q] = src.f[q][id];
   if constexpr (NC == 2) {
@@ -329,23 +310,19 @@ __device__ __forceinline__ void load_src(const CFields &src, const float *__rest
     s.nn[0] = n0a[id];
     s.nn[1] = n1a[id];
     s.nn[2] = n2a[id];
+    s.kap = ka[id];
   }
 }
 
 template <int NC, bool ADD>
-__device__ __forceinline__ void eval_src(const Src<NC> &s, const float *__restrict__ n0a,
-                                         const float *__restrict__ n1a,
-                                         const float *__restrict__ n2a, const Params &p, int gx,
-                                         int gy, int gz, PopSink &h, float *gq) {
+__device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, PopSink &h, float *gq) {
   const float *m = s.m;
   if constexpr (NC == 2) {
-    const int plane = p.nx * p.ny;
     const float phi = m[10];
     const float n0 = s.nn[0], n1 = s.nn[1], n2 = s.nn[2];
-    // CSF curvature shift, recursive_2c_TSLB3D.f90:614-636
-    float curv = 0.f;
-    const float ind = phi * (1.f - phi) * (n0 * n0 + n1 * n1 + n2 * n2);
-    if (ind > 0.2f) curv = curvature_at(n0a, n1a, n2a, p, gx, gy, gz);
+    // CSF velocity shift sigma*kappa*n, recursive_2c_TSLB3D.f90:614-636; the
+    // gated curvature comes from the normals pass
+    const float curv = s.kap;
     const float sx = p.sigma * curv * n0, sy = p.sigma * curv * n1, sz = p.sigma * curv * n2;
     const float esx = sx + p.fx + p.lin_a * m[1];
     const float esy = sy + p.fy + p.lin_a * m[2];
@@ -461,6 +438,7 @@ template <int NC, bool MASK, bool ADD, bool PF, int WPE>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(CFields src, Fields dst, const float *__restrict__ n0a,
                                                 const float *__restrict__ n1a,
                                                 const float *__restrict__ n2a,
+                                                const float *__restrict__ ka,
                                                 const uint32_t *__restrict__ links, Params p) {
   constexpr int NQ = NC == 2 ? 28 : 24;
   __shared__ float lds[NQ][NSLOT];
@@ -526,13 +504,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     if (ook) {
       const int id = ox + p.nx * oy + plane * gz;
       if constexpr (MASK) ook = (links[id] & kSolid) == 0u;
-      if (ook) load_src<NC>(src, n0a, n1a, n2a, id, o);
+      if (ook) load_src<NC>(src, n0a, n1a, n2a, ka, id, o);
     }
     rok = zok && rxy;
     if (rok) {
       const int id = rx + p.nx * ry + plane * gz;
       if constexpr (MASK) rok = (links[id] & kSolid) == 0u;
-      if (rok) load_src<NC>(src, n0a, n1a, n2a, id, r);
+      if (rok) load_src<NC>(src, n0a, n1a, n2a, ka, id, r);
     }
   };
   if constexpr (PF) fetch(z0 - 1, so, so_ok, sr, sr_ok);
@@ -561,7 +539,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     // ---- own source slot
     if (co_ok) {
       PopSink sk{lds, (ty + 1) * SW + (tx + 1), 0.f, 0.f, 0.f};
-      eval_src<NC, ADD>(co, n0a, n1a, n2a, p, ox, oy, gz, sk, gq);
+      eval_src<NC, ADD>(co, p, sk, gq);
       h0 = sk.h0;
       h5 = sk.h5;
       h6 = sk.h6;
@@ -574,7 +552,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     // ---- halo ring slot
     if (cr_ok) {
       PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
-      eval_src<NC, ADD>(cr, n0a, n1a, n2a, p, rx, ry, gz, sk, gq);
+      eval_src<NC, ADD>(cr, p, sk, gq);
     }
     __syncthreads();
 
@@ -651,52 +629,23 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   }
 }
 
-// ---------------------------------------------------------------------------
-// K2: normals n = grad(phi)/(|grad(phi)| + 1e-9) (recursive_2c_TSLB3D.f90:
-// 555-571); phi read across a non-periodic face at the nearest interior node;
-// zero on solid nodes.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_normals(const float *__restrict__ phi,
-                                                 float *__restrict__ n0,
-                                                 float *__restrict__ n1,
-                                                 float *__restrict__ n2,
-                                                 const uint32_t *__restrict__ links, int has_mask,
-                                                 Params p) {
-  const int n = p.nx * p.ny * p.nz;
-  const int id = blockIdx.x * 256 + threadIdx.x;
-  if (id >= n) return;
-  if (has_mask && (links[id] & kSolid)) {
-    n0[id] = n1[id] = n2[id] = 0.f;
-    return;
-  }
-  const int x = id % p.nx;
-  const int y = (id / p.nx) % p.ny;
-  const int z = id / (p.nx * p.ny);
-  auto ph = [&](int dx, int dy, int dz) -> float {
-    int xx = x + dx, yy = y + dy, zz = z + dz;
-    if (xx < 0 || xx >= p.nx) xx = p.px ? wrapc(xx, p.nx) : (xx < 0 ? 0 : p.nx - 1);
-    if (yy < 0 || yy >= p.ny) yy = p.py ? wrapc(yy, p.ny) : (yy < 0 ? 0 : p.ny - 1);
-    if (zz < 0 || zz >= p.nz) zz = p.pz ? wrapc(zz, p.nz) : (zz < 0 ? 0 : p.nz - 1);
-    return phi[xx + p.nx * yy + p.nx * p.ny * zz];
-  };
-  const float gx = grad_axis<0>(ph), gy = grad_axis<1>(ph), gz = grad_axis<2>(ph);
-  const float mod = sqrtf(gx * gx + gy * gy + gz * gz);
-  const float den = mod + 1e-9f;
-  n0[id] = gx / den;
-  n1[id] = gy / den;
-  n2[id] = gz / den;
-}
-
-// K2, LDS-tiled: a 64x8 block marches a z-chunk, keeping three phi planes of
-// its (64+2)x(8+2) footprint in LDS; one global read of phi per node (plus the
-// footprint ring) instead of 19.
+// K2 + curvature, LDS-tiled.  A 64x8 block marches a z-chunk.  It keeps five
+// phi planes of its (64+4)x(8+4) footprint and three normal planes of its
+// (64+2)x(8+2) footprint in LDS, writes n = grad(phi)/(|grad(phi)|+1e-9)
+// (recursive_2c_TSLB3D.f90:555-571) and kappa = -div(n) at nodes past the
+// interface gate phi(1-phi)|n|^2 > 0.2 (0 elsewhere, :614-631).  Across a
+// non-periodic face phi is read at the nearest interior node and n is 0; n is
+// 0 on solid nodes.
+constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
 __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict__ phi,
                                                          float *__restrict__ n0,
                                                          float *__restrict__ n1,
                                                          float *__restrict__ n2,
+                                                         float *__restrict__ kap,
                                                          const uint32_t *__restrict__ links,
                                                          int has_mask, Params p) {
-  __shared__ float ph[3][NSLOT];
+  __shared__ float ph[5][NPSLOT];
+  __shared__ float nl[3][3][NSLOT];
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   int b = blockIdx.x;
@@ -714,38 +663,74 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
     if (g >= 0 && g < n) return g;
     return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
   };
-  auto load_plane = [&](int z) {
+  auto mod3 = [](int z) { return ((z % 3) + 3) % 3; };
+  auto mod5 = [](int z) { return ((z % 5) + 5) % 5; };
+  auto load_phi = [&](int z) {
     const int zz = clampw(z, p.nz, p.pz);
-    float *dstp = ph[((z % 3) + 3) % 3];
-    for (int sl = tid; sl < NSLOT; sl += NTHR) {
-      const int sx = sl % SW, sy = sl / SW;
-      const int gx = clampw(x0 + sx - 1, p.nx, p.px);
-      const int gy = clampw(y0 + sy - 1, p.ny, p.py);
+    float *dstp = ph[mod5(z)];
+    for (int sl = tid; sl < NPSLOT; sl += NTHR) {
+      const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
+      const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
       dstp[sl] = phi[gx + p.nx * gy + plane * zz];
     }
   };
-  load_plane(z0 - 1);
-  load_plane(z0);
+  // normals of plane z on the (TX+2)x(TY+2) footprint, from phi planes z-1..z+1
+  auto normals_plane = [&](int z) {
+    int gz;
+    const bool zin = src_coord(z, p.nz, p.pz, gz);
+    for (int sl = tid; sl < NSLOT; sl += NTHR) {
+      const int sx = sl % SW, sy = sl / SW;
+      int gx, gy;
+      bool ok = zin && src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
+      if (ok && has_mask) ok = (links[gx + p.nx * gy + plane * gz] & kSolid) == 0u;
+      float a = 0.f, bb = 0.f, c = 0.f;
+      if (ok) {
+        auto get = [&](int dx, int dy, int dz) -> float {
+          return ph[mod5(z + dz)][(sy + 1 + dy) * PW + (sx + 1 + dx)];
+        };
+        const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
+        const float den = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f;
+        a = gx_ / den;
+        bb = gy_ / den;
+        c = gz_ / den;
+      }
+      nl[mod3(z)][0][sl] = a;
+      nl[mod3(z)][1][sl] = bb;
+      nl[mod3(z)][2][sl] = c;
+    }
+  };
+  for (int z = z0 - 2; z <= z0 + 1; z++) load_phi(z);
+  __syncthreads();
+  normals_plane(z0 - 1);
   for (int z = z0; z < z1; z++) {
-    load_plane(z + 1);
+    load_phi(z + 2);
+    __syncthreads();
+    normals_plane(z + 1);
+    if (z == z0) normals_plane(z0);
     __syncthreads();
     if (active) {
       const int id = x + p.nx * y + plane * z;
-      if (has_mask && (links[id] & kSolid)) {
-        n0[id] = n1[id] = n2[id] = 0.f;
-      } else {
-        auto get = [&](int dx, int dy, int dz) -> float {
-          return ph[(((z + dz) % 3) + 3) % 3][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+      const int own = (ty + 1) * SW + (tx + 1);
+      const float a = nl[mod3(z)][0][own], bb = nl[mod3(z)][1][own], c = nl[mod3(z)][2][own];
+      n0[id] = a;
+      n1[id] = bb;
+      n2[id] = c;
+      const float f = ph[mod5(z)][(ty + 2) * PW + (tx + 2)];
+      float curv = 0.f;
+      const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
+      if (ind > 0.2f) {
+        auto nat = [&](int comp) {
+          return [&, comp](int dx, int dy, int dz) -> float {
+            return nl[mod3(z + dz)][comp][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+          };
         };
-        const float gx = grad_axis<0>(get), gy = grad_axis<1>(get), gz = grad_axis<2>(get);
-        const float mod = sqrtf(gx * gx + gy * gy + gz * gz);
-        const float den = mod + 1e-9f;
-        n0[id] = gx / den;
-        n1[id] = gy / den;
-        n2[id] = gz / den;
+        const float dxx = grad_axis<0>(nat(0));
+        const float dyy = grad_axis<1>(nat(1));
+        const float dzz = grad_axis<2>(nat(2));
+        curv = -(dxx + dyy + dzz);
       }
+      kap[id] = curv;
     }
-    __syncthreads();
   }
 }
 
@@ -1001,8 +986,7 @@ struct tslb_ctx {
   double *part = nullptr, *red = nullptr;
   float *w_in = nullptr, *phi_in = nullptr;
   int zbc = 0, zbc_phi = 0;
-  int normals_variant = 0;  // 0: LDS-tiled, 1: one thread per node through L1
-  int variant = 1;  // bit 0: prefetch the next source plane; bit 1: cap VGPRs for 2 blocks/CU
+  int variant = 3;  // bit 0: prefetch the next source plane; bit 1: cap VGPRs for 2 blocks/CU
   size_t bytes = 0;
   int timing = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -1071,14 +1055,10 @@ int launch_normals(tslb_ctx *c) {
     e1 = take_event(c);
     HIPCK(hipEventRecord(e0, c->stream));
   }
-  if (c->normals_variant == 0) {
-    const int nblk = c->kp.tiles_x * c->kp.tiles_y * c->kp.nzc;
-    hipLaunchKernelGGL(k_normals_tiled, dim3(nblk), dim3(NTHR), 0, c->stream, base + TSLB_PHI * c->fs,
-                       c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->links, c->has_mask, c->kp);
-  } else {
-    hipLaunchKernelGGL(k_normals, dim3(nb), dim3(256), 0, c->stream, base + TSLB_PHI * c->fs,
-                       c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->links, c->has_mask, c->kp);
-  }
+  const int nblk = c->kp.tiles_x * c->kp.tiles_y * c->kp.nzc;
+  hipLaunchKernelGGL(k_normals_tiled, dim3(nblk), dim3(NTHR), 0, c->stream, base + TSLB_PHI * c->fs,
+                     c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->links,
+                     c->has_mask, c->kp);
   HIPCK(hipGetLastError());
   if (c->timing) {
     HIPCK(hipEventRecord(e1, c->stream));
@@ -1095,7 +1075,7 @@ int launch_fused(tslb_ctx *c) {
   const int nblk = p.tiles_x * p.tiles_y * p.nzc;
   CFields s = cfields_of(src, c->fs);
   Fields d = fields_of(dst, c->fs);
-  const float *n0 = c->nrm, *n1 = c->nrm + c->fs, *n2 = c->nrm + 2 * c->fs;
+  const float *n0 = c->nrm, *n1 = c->nrm + c->fs, *n2 = c->nrm + 2 * c->fs, *ka = c->nrm + 3 * c->fs;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = take_event(c);
@@ -1108,7 +1088,7 @@ int launch_fused(tslb_ctx *c) {
   const bool w4 = (c->variant & 2) != 0;
 #define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                  \
   hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
-                     n0, n1, n2, c->links, p)
+                     n0, n1, n2, ka, c->links, p)
 #define TSLB_LAUNCH_V(NC, MASK, ADD)                 \
   do {                                               \
     if (pf && w4) TSLB_LAUNCH(NC, MASK, ADD, true, 4);   \
@@ -1223,7 +1203,6 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
 
   if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 3;
-  if (const char *v = getenv("TSLB_NORMALS_VARIANT")) c->normals_variant = atoi(v) & 1;
   int rc = TSLB_OK;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1232,7 +1211,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   }
   const size_t fb = c->fs * TSLB_NFIELDS * sizeof(float);
   if ((rc = dev_alloc(c, (void **)&c->buf[0], fb)) || (rc = dev_alloc(c, (void **)&c->buf[1], fb)) ||
-      (p->ncomp == 2 && (rc = dev_alloc(c, (void **)&c->nrm, 3 * c->fs * sizeof(float)))) ||
+      (p->ncomp == 2 && (rc = dev_alloc(c, (void **)&c->nrm, 4 * c->fs * sizeof(float)))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
       (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
     std::string msg = g_err;
@@ -1245,7 +1224,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->buf[b], 0.f, tot);
   }
   if (c->nrm) {
-    const int tot = (int)(3 * c->fs);
+    const int tot = (int)(4 * c->fs);
     hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->nrm, 0.f, tot);
   }
   e = hipStreamSynchronize(c->stream);
