@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--n", type=int, default=512, help="lattice edge (512 for the named workload)")
     ap.add_argument("--workload", default="emulsion", choices=("emulsion", "channel", "tgv"),
                     help="emulsion (2C, the named workload), channel (1C, n x n/2 x n/2), tgv (1C, n^3)")
+    ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16", "bf16"),
+                    help="moment storage precision (compute is FP32)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -107,7 +109,7 @@ def main():
     ncomp = case.lattice["ncomp"]
     log(f"bench: {args.workload} {nx}x{ny}x{nz} set up in {time.time() - t0:.1f} s ({case.notes})")
     nodes = nx * ny * nz
-    with Lattice(nx, ny, nz, device=0, **case.lattice) as lat:
+    with Lattice(nx, ny, nz, device=0, storage=args.storage, **case.lattice) as lat:
         case.apply(lat)
         log(f"bench: state on device ({lat.device_bytes() / 2**30:.2f} GiB)")
         copy_gbs = lat.stream_copy_gbs()
@@ -131,7 +133,8 @@ def main():
 
     mlups = nodes * args.steps / wall / 1e6
     nfields = 11 if ncomp == 2 else 10
-    bytes_per_launch = 8.0 * nfields * nodes          # read + write of the FP32 moment fields
+    es = 4 if args.storage == "fp32" else 2
+    bytes_per_launch = 2.0 * es * nfields * nodes     # read + write of the stored moment fields
     t_fused = fused_ms / max(fused_n, 1) / 1e3
     achieved = bytes_per_launch / t_fused / 1e9
     t_step = (fused_ms + norm_ms) / max(fused_n, 1) / 1e3
@@ -147,10 +150,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.storage == "fp32" else f"f32 compute, {args.storage} storage",
         "data": "synthetic (seeded droplets and velocity noise)",
         "config": {"workload": f"{args.workload}-{ncomp}c-{nx}x{ny}x{nz}", "nx": nx, "ny": ny, "nz": nz,
-                   "components": ncomp, "storage": "fp32", "notes": case.notes,
+                   "components": ncomp, "storage": args.storage, "notes": case.notes,
                    **{k: v for k, v in case.lattice.items() if k in ("tau", "sigma", "sharp_c", "lin_force", "force")}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -160,7 +163,7 @@ def main():
         "hbm_stream_copy_gbs": round(copy_gbs, 1),
         "fused_frac_of_copy": round(achieved / copy_gbs, 4),
     }
-    if not args.no_cpu and args.workload == "emulsion":
+    if not args.no_cpu and args.workload == "emulsion" and args.storage == "fp32":
         try:
             out["cpu_baseline"] = cpu_baseline(n, case, args.cpu_budget)
         except Exception as e:  # the baseline is reported, never the product

@@ -35,7 +35,10 @@ def build_library(verbose=False):
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), file=sys.stderr)
-    subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError("hipcc failed building libtslb.so")
     os.replace(tmp, LIB)
     return LIB
 

@@ -98,18 +98,69 @@ struct Params {
   float sigma, sharp_c;
   float fx, fy, fz, lin_a;
   int z_chunk, tiles_x, tiles_y, nzc;
+  float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
+  float isc[TSLB_NFIELDS];  // 1 / sc
 };
 
-struct Fields {
-  float *f[TSLB_NFIELDS];
+
+// Moment storage: FP32, FP16 or BF16 (round-to-nearest-even).  Reduced
+// storage keeps every field pre-multiplied by a per-field scale (Params::sc)
+// so that the shifted density and Pi^neq sit in FP16's normal range; compute
+// is always FP32.
+struct bf16_t {
+  uint16_t b;
 };
-struct CFields {
-  const float *f[TSLB_NFIELDS];
+__device__ __forceinline__ float to_f(float x) { return x; }
+__device__ __forceinline__ float to_f(_Float16 x) { return (float)x; }
+__device__ __forceinline__ float to_f(bf16_t x) { return __uint_as_float((uint32_t)x.b << 16); }
+template <typename S>
+__device__ __forceinline__ S from_f(float v);
+template <>
+__device__ __forceinline__ float from_f<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ _Float16 from_f<_Float16>(float v) { return (_Float16)v; }
+template <>
+__device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return bf16_t{(uint16_t)(u >> 16)};
+}
+
+template <typename S>
+struct FieldsT {
+  S *f[TSLB_NFIELDS];
 };
+template <typename S>
+struct CFieldsT {
+  const S *f[TSLB_NFIELDS];
+};
+
+// scaled field access
+template <typename S>
+__device__ __forceinline__ float ldf(const S *__restrict__ a, int id, const Params &p, int q) {
+  if constexpr (sizeof(S) == 4) return to_f(a[id]);
+  else return to_f(a[id]) * p.isc[q];
+}
+template <typename S>
+__device__ __forceinline__ void stf(S *__restrict__ a, int id, float v, const Params &p, int q) {
+  if constexpr (sizeof(S) == 4) a[id] = from_f<S>(v);
+  else a[id] = from_f<S>(v * p.sc[q]);
+}
 
 __device__ __forceinline__ int wrapc(int g, int n) {
   int r = g % n;
   return r < 0 ? r + n : r;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Map dispatch order to tiles so that consecutive tiles (which share
+// halo rows and columns) run on the same XCD at about the same time: XCD x
+// gets the contiguous tile range [x*nblk/8, (x+1)*nblk/8).
+__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+  constexpr int NXCD = 8;
+  const int per = nblk / NXCD;
+  if (b >= per * NXCD) return b;  // remainder blocks keep their order
+  return (b % NXCD) * per + b / NXCD;
 }
 
 // coordinate of a source along one axis; false beyond a non-periodic face
@@ -298,15 +349,16 @@ struct Src {
   float kap;                   // gated curvature (two components)
 };
 
-template <int NC>
-__device__ __forceinline__ void load_src(const CFields &src, const float *__restrict__ n0a,
+template <int NC, typename S>
+__device__ __forceinline__ void load_src(const CFieldsT<S> &src, const float *__restrict__ n0a,
                                          const float *__restrict__ n1a,
                                          const float *__restrict__ n2a,
-                                         const float *__restrict__ ka, int id, Src<NC> &s) {
+                                         const float *__restrict__ ka, int id, const Params &p,
+                                         Src<NC> &s) {
 #pragma unroll
-  for (int q = 0; q < 10; q++) s.m[q] = src.f[q][id];
+  for (int q = 0; q < 10; q++) s.m[q] = ldf(src.f[q], id, p, q);
   if constexpr (NC == 2) {
-    s.m[10] = src.f[TSLB_PHI][id];
+    s.m[10] = ldf(src.f[TSLB_PHI], id, p, TSLB_PHI);
     s.nn[0] = n0a[id];
     s.nn[1] = n1a[id];
     s.nn[2] = n2a[id];
@@ -434,8 +486,8 @@ __device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int 
   acc_layer<CZ>(a, p);
 }
 
-template <int NC, bool MASK, bool ADD, bool PF, int WPE>
-__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(CFields src, Fields dst, const float *__restrict__ n0a,
+template <int NC, bool MASK, bool ADD, bool PF, int WPE, typename S>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(CFieldsT<S> src, FieldsT<S> dst, const float *__restrict__ n0a,
                                                 const float *__restrict__ n1a,
                                                 const float *__restrict__ n2a,
                                                 const float *__restrict__ ka,
@@ -445,7 +497,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  int b = blockIdx.x;
+  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int tile_x = b % p.tiles_x;
   b /= p.tiles_x;
   const int tile_y = b % p.tiles_y;
@@ -504,13 +556,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     if (ook) {
       const int id = ox + p.nx * oy + plane * gz;
       if constexpr (MASK) ook = (links[id] & kSolid) == 0u;
-      if (ook) load_src<NC>(src, n0a, n1a, n2a, ka, id, o);
+      if (ook) load_src<NC>(src, n0a, n1a, n2a, ka, id, p, o);
     }
     rok = zok && rxy;
     if (rok) {
       const int id = rx + p.nx * ry + plane * gz;
       if constexpr (MASK) rok = (links[id] & kSolid) == 0u;
-      if (rok) load_src<NC>(src, n0a, n1a, n2a, ka, id, r);
+      if (rok) load_src<NC>(src, n0a, n1a, n2a, ka, id, p, r);
     }
   };
   if constexpr (PF) fetch(z0 - 1, so, so_ok, sr, sr_ok);
@@ -608,17 +660,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         const float inv = 1.f / rho;
         const float u = aA.sx * inv, v = aA.sy * inv, w = aA.sz * inv;
         const float t = aA.s0 * (1.f / 3.f);
-        dst.f[0][id] = aA.s0;
-        dst.f[1][id] = u;
-        dst.f[2][id] = v;
-        dst.f[3][id] = w;
-        dst.f[4][id] = aA.sxx - t - rho * u * u;
-        dst.f[5][id] = aA.syy - t - rho * v * v;
-        dst.f[6][id] = aA.szz - t - rho * w * w;
-        dst.f[7][id] = aA.sxy - rho * u * v;
-        dst.f[8][id] = aA.sxz - rho * u * w;
-        dst.f[9][id] = aA.syz - rho * v * w;
-        if constexpr (NC == 2) dst.f[10][id] = aA.phi;
+        stf(dst.f[0], id, aA.s0, p, 0);
+        stf(dst.f[1], id, u, p, 1);
+        stf(dst.f[2], id, v, p, 2);
+        stf(dst.f[3], id, w, p, 3);
+        stf(dst.f[4], id, aA.sxx - t - rho * u * u, p, 4);
+        stf(dst.f[5], id, aA.syy - t - rho * v * v, p, 5);
+        stf(dst.f[6], id, aA.szz - t - rho * w * w, p, 6);
+        stf(dst.f[7], id, aA.sxy - rho * u * v, p, 7);
+        stf(dst.f[8], id, aA.sxz - rho * u * w, p, 8);
+        stf(dst.f[9], id, aA.syz - rho * v * w, p, 9);
+        if constexpr (NC == 2) stf(dst.f[10], id, aA.phi, p, 10);
       }
     }
     __syncthreads();
@@ -637,7 +689,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 // non-periodic face phi is read at the nearest interior node and n is 0; n is
 // 0 on solid nodes.
 constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
-__global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict__ phi,
+template <typename S>
+__global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ phi,
                                                          float *__restrict__ n0,
                                                          float *__restrict__ n1,
                                                          float *__restrict__ n2,
@@ -648,7 +701,7 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
   __shared__ float nl[3][3][NSLOT];
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  int b = blockIdx.x;
+  int b = xcd_tile(blockIdx.x, gridDim.x);
   const int tile_x = b % p.tiles_x;
   b /= p.tiles_x;
   const int tile_y = b % p.tiles_y;
@@ -663,15 +716,16 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
     if (g >= 0 && g < n) return g;
     return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
   };
-  auto mod3 = [](int z) { return ((z % 3) + 3) % 3; };
-  auto mod5 = [](int z) { return ((z % 5) + 5) % 5; };
+  // ring slots are indexed by (z - z0 + 6) mod 3 / mod 5 with z >= z0 - 2
+  auto mod3 = [z0](int z) { return (z - z0 + 6) % 3; };
+  auto mod5 = [z0](int z) { return (z - z0 + 10) % 5; };
   auto load_phi = [&](int z) {
     const int zz = clampw(z, p.nz, p.pz);
     float *dstp = ph[mod5(z)];
     for (int sl = tid; sl < NPSLOT; sl += NTHR) {
       const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
       const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
-      dstp[sl] = phi[gx + p.nx * gy + plane * zz];
+      dstp[sl] = ldf(phi, gx + p.nx * gy + plane * zz, p, TSLB_PHI);
     }
   };
   // normals of plane z on the (TX+2)x(TY+2) footprint, from phi planes z-1..z+1
@@ -685,8 +739,9 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
       if (ok && has_mask) ok = (links[gx + p.nx * gy + plane * gz] & kSolid) == 0u;
       float a = 0.f, bb = 0.f, c = 0.f;
       if (ok) {
+        const float *pl[3] = {ph[mod5(z - 1)], ph[mod5(z)], ph[mod5(z + 1)]};
         auto get = [&](int dx, int dy, int dz) -> float {
-          return ph[mod5(z + dz)][(sy + 1 + dy) * PW + (sx + 1 + dx)];
+          return pl[dz + 1][(sy + 1 + dy) * PW + (sx + 1 + dx)];
         };
         const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
         const float den = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f;
@@ -694,9 +749,10 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
         bb = gy_ / den;
         c = gz_ / den;
       }
-      nl[mod3(z)][0][sl] = a;
-      nl[mod3(z)][1][sl] = bb;
-      nl[mod3(z)][2][sl] = c;
+      const int zs3 = mod3(z);
+      nl[zs3][0][sl] = a;
+      nl[zs3][1][sl] = bb;
+      nl[zs3][2][sl] = c;
     }
   };
   for (int z = z0 - 2; z <= z0 + 1; z++) load_phi(z);
@@ -711,7 +767,8 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
     if (active) {
       const int id = x + p.nx * y + plane * z;
       const int own = (ty + 1) * SW + (tx + 1);
-      const float a = nl[mod3(z)][0][own], bb = nl[mod3(z)][1][own], c = nl[mod3(z)][2][own];
+      const int zm = mod3(z - 1), z0s = mod3(z), zp = mod3(z + 1);
+      const float a = nl[z0s][0][own], bb = nl[z0s][1][own], c = nl[z0s][2][own];
       n0[id] = a;
       n1[id] = bb;
       n2[id] = c;
@@ -721,7 +778,8 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
       if (ind > 0.2f) {
         auto nat = [&](int comp) {
           return [&, comp](int dx, int dy, int dz) -> float {
-            return nl[mod3(z + dz)][comp][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+            const int zz = dz < 0 ? zm : (dz > 0 ? zp : z0s);
+            return nl[zz][comp][(ty + 1 + dy) * SW + (tx + 1 + dx)];
           };
         };
         const float dxx = grad_axis<0>(nat(0));
@@ -734,11 +792,21 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const float *__restrict_
   }
 }
 
-// plain copy for bandwidth calibration (float4, grid-stride)
-__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b,
+// plain copy for bandwidth calibration: each thread moves 4 float4 per
+// iteration (16 B per lane per instruction, fully coalesced)
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(const v4f *__restrict__ a, v4f *__restrict__ b,
                                               long long n4) {
-  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n4; i += (long long)gridDim.x * 256)
-    b[i] = a[i];
+  const long long stride = (long long)gridDim.x * 1024;
+  for (long long base = blockIdx.x * 1024LL + threadIdx.x; base < n4; base += stride) {
+    v4f v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (base + k * 256 < n4) v[k] = __builtin_nontemporal_load(a + base + k * 256);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (base + k * 256 < n4) __builtin_nontemporal_store(v[k], b + base + k * 256);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -746,11 +814,12 @@ __global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a, floa
 // (recursive_2c_TSLB3D.f90:256-283 writes g(i+1,..,1) etc.), bounce-back at
 // invalid sources.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prepush(const float *__restrict__ phi,
-                                                 const float *__restrict__ u,
-                                                 const float *__restrict__ v,
-                                                 const float *__restrict__ w,
-                                                 float *__restrict__ out,
+template <typename S>
+__global__ __launch_bounds__(256) void k_prepush(const S *__restrict__ phi,
+                                                 const S *__restrict__ u,
+                                                 const S *__restrict__ v,
+                                                 const S *__restrict__ w,
+                                                 S *__restrict__ out,
                                                  const uint32_t *__restrict__ links, int has_mask,
                                                  Params p) {
   const int n = p.nx * p.ny * p.nz;
@@ -763,12 +832,14 @@ __global__ __launch_bounds__(256) void k_prepush(const float *__restrict__ phi,
   const int x = id % p.nx;
   const int y = (id / p.nx) % p.ny;
   const int z = id / (p.nx * p.ny);
-  const float *vel[3] = {u, v, w};
+  const S *vel[3] = {u, v, w};
   float own[7];
-  own[0] = 0.25f * phi[id];
+  const float ph0 = ldf(phi, id, p, TSLB_PHI);
+  own[0] = 0.25f * ph0;
   for (int a = 0; a < 3; a++) {
-    own[1 + 2 * a] = 0.125f * phi[id] * (1.f + 3.f * vel[a][id]);
-    own[2 + 2 * a] = 0.125f * phi[id] * (1.f - 3.f * vel[a][id]);
+    const float ua = ldf(vel[a], id, p, TSLB_U + a);
+    own[1 + 2 * a] = 0.125f * ph0 * (1.f + 3.f * ua);
+    own[2 + 2 * a] = 0.125f * ph0 * (1.f - 3.f * ua);
   }
   float s = own[0];
   for (int l = 1; l <= 6; l++) {
@@ -782,10 +853,10 @@ __global__ __launch_bounds__(256) void k_prepush(const float *__restrict__ phi,
     } else {
       const int a = (l - 1) / 2;
       const float sgn = (l & 1) ? 1.f : -1.f;
-      s += 0.125f * phi[sid] * (1.f + sgn * 3.f * vel[a][sid]);
+      s += 0.125f * ldf(phi, sid, p, TSLB_PHI) * (1.f + sgn * 3.f * ldf(vel[a], sid, p, TSLB_U + a));
     }
   }
-  out[id] = s;
+  stf(out, id, s, p, TSLB_PHI);
 }
 
 // ---------------------------------------------------------------------------
@@ -819,7 +890,8 @@ __global__ __launch_bounds__(256) void k_links(const int32_t *__restrict__ isf,
 // Moment-space z-plane BC (TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292):
 // plane 0 <- plane 1 with u = v = 0 and w = w_in; plane nz-1 <- plane nz-2.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_zplane_bc(Fields f, const float *__restrict__ w_in,
+template <typename S>
+__global__ __launch_bounds__(256) void k_zplane_bc(FieldsT<S> f, const float *__restrict__ w_in,
                                                    const float *__restrict__ phi_in, int ncomp,
                                                    Params p) {
   const int plane = p.nx * p.ny;
@@ -830,9 +902,9 @@ __global__ __launch_bounds__(256) void k_zplane_bc(Fields f, const float *__rest
   const int nf = ncomp == 2 ? TSLB_NFIELDS : 10;
   for (int q = 0; q < nf; q++) f.f[q][hi] = f.f[q][hi1];
   for (int q = 0; q < nf; q++) {
-    if (q == TSLB_U || q == TSLB_V) f.f[q][lo] = 0.f;
-    else if (q == TSLB_W) f.f[q][lo] = w_in[id];
-    else if (q == TSLB_PHI && phi_in) f.f[q][lo] = phi_in[id];
+    if (q == TSLB_U || q == TSLB_V) stf(f.f[q], lo, 0.f, p, q);
+    else if (q == TSLB_W) stf(f.f[q], lo, w_in[id], p, q);
+    else if (q == TSLB_PHI && phi_in) stf(f.f[q], lo, phi_in[id], p, q);
     else f.f[q][lo] = f.f[q][lo1];
   }
 }
@@ -872,7 +944,8 @@ __host__ __device__ inline float rand_noseeded_impl(int i, int j, int k, int l) 
 
 // field += amp * (2 r - 1), r = rand_noseeded(i, j, k, l) with 1-based i, j, k
 // (seeding as TSLB_2C-mpi/lb_kernels_module.f90:970-977)
-__global__ __launch_bounds__(256) void k_seed_noise(float *__restrict__ a, float amp, int l, Params p) {
+template <typename S>
+__global__ __launch_bounds__(256) void k_seed_noise(S *__restrict__ a, int q, float amp, int l, Params p) {
   const int n = p.nx * p.ny * p.nz;
   const int id = blockIdx.x * 256 + threadIdx.x;
   if (id >= n) return;
@@ -880,7 +953,25 @@ __global__ __launch_bounds__(256) void k_seed_noise(float *__restrict__ a, float
   const int y = (id / p.nx) % p.ny;
   const int z = id / (p.nx * p.ny);
   const float r = rand_noseeded_impl(x + 1, y + 1, z + 1, l);
-  a[id] = a[id] + amp * (2.0f * r - 1.0f);
+  stf(a, id, ldf(a, id, p, q) + amp * (2.0f * r - 1.0f), p, q);
+}
+
+// host-format float <-> storage, with the field's scale; field 0 carries rho-1
+template <typename S>
+__global__ __launch_bounds__(256) void k_to_storage(const float *__restrict__ in, S *__restrict__ out,
+                                                    int n, int q, Params p) {
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= n) return;
+  const float v = q == TSLB_RHO ? in[id] - 1.0f : in[id];
+  stf(out, id, v, p, q);
+}
+template <typename S>
+__global__ __launch_bounds__(256) void k_from_storage(const S *__restrict__ in, float *__restrict__ out,
+                                                      int n, int q, Params p) {
+  const int id = blockIdx.x * 256 + threadIdx.x;
+  if (id >= n) return;
+  const float v = ldf(in, id, p, q);
+  out[id] = q == TSLB_RHO ? 1.0f + v : v;
 }
 
 __global__ __launch_bounds__(256) void k_fill(float *__restrict__ a, float v, int n) {
@@ -892,7 +983,8 @@ __global__ __launch_bounds__(256) void k_fill(float *__restrict__ a, float v, in
 // FP64 diagnostics: per-block partial sums, then one block sums the partials.
 // ---------------------------------------------------------------------------
 constexpr int NRED = 7;
-__global__ __launch_bounds__(256) void k_reduce_part(CFields f, const uint32_t *__restrict__ links,
+template <typename S>
+__global__ __launch_bounds__(256) void k_reduce_part(CFieldsT<S> f, const uint32_t *__restrict__ links,
                                                      int has_mask, Params p,
                                                      double *__restrict__ part) {
   __shared__ double sh[NRED][256];
@@ -903,10 +995,10 @@ __global__ __launch_bounds__(256) void k_reduce_part(CFields f, const uint32_t *
     const int x = id % p.nx;
     const int y = (id / p.nx) % p.ny;
     const int z = id / (p.nx * p.ny);
-    const double rho = 1.0 + (double)f.f[0][id];
-    const double u = f.f[1][id], v = f.f[2][id], w = f.f[3][id];
+    const double rho = 1.0 + (double)ldf(f.f[0], id, p, 0);
+    const double u = ldf(f.f[1], id, p, 1), v = ldf(f.f[2], id, p, 2), w = ldf(f.f[3], id, p, 3);
     acc[0] += rho;
-    if (p.ncomp == 2) acc[1] += (double)f.f[TSLB_PHI][id];
+    if (p.ncomp == 2) acc[1] += (double)ldf(f.f[TSLB_PHI], id, p, TSLB_PHI);
     acc[2] += 0.5 * rho * (u * u + v * v + w * w);
     acc[3] += rho * u;
     acc[4] += rho * v;
@@ -916,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_reduce_part(CFields f, const uint32_t *
       if (xx < 0 || xx >= p.nx) xx = p.px ? wrapc(xx, p.nx) : (xx < 0 ? 0 : p.nx - 1);
       if (yy < 0 || yy >= p.ny) yy = p.py ? wrapc(yy, p.ny) : (yy < 0 ? 0 : p.ny - 1);
       if (zz < 0 || zz >= p.nz) zz = p.pz ? wrapc(zz, p.nz) : (zz < 0 ? 0 : p.nz - 1);
-      return (double)f.f[q][xx + p.nx * yy + p.nx * p.ny * zz];
+      return (double)ldf(f.f[q], xx + p.nx * yy + p.nx * p.ny * zz, p, q);
     };
     const double wx = 0.5 * (at(3, 0, 1, 0) - at(3, 0, -1, 0)) - 0.5 * (at(2, 0, 0, 1) - at(2, 0, 0, -1));
     const double wy = 0.5 * (at(1, 0, 0, 1) - at(1, 0, 0, -1)) - 0.5 * (at(3, 1, 0, 0) - at(3, -1, 0, 0));
@@ -977,9 +1069,10 @@ struct tslb_ctx {
   Params kp;
   hipStream_t stream = nullptr;
   size_t n = 0, fs = 0;  // nodes, field stride (elements)
-  float *buf[2] = {nullptr, nullptr};
+  size_t es = 4;         // bytes per stored moment value
+  void *buf[2] = {nullptr, nullptr};
   int cur = 0;
-  float *nrm = nullptr;  // 3 * fs
+  float *nrm = nullptr;  // n0, n1, n2, kappa: 4 * fs floats
   uint32_t *links = nullptr;
   int has_mask = 0;
   int normals_fresh = 0;
@@ -997,14 +1090,30 @@ struct tslb_ctx {
 
 namespace {
 
-Fields fields_of(float *base, size_t fs) {
-  Fields f;
-  for (int q = 0; q < TSLB_NFIELDS; q++) f.f[q] = base + (size_t)q * fs;
+// call fn with a value of the storage type of the context
+template <typename Fn>
+auto with_storage(const tslb_ctx *c, Fn &&fn) {
+  switch (c->prm.storage) {
+    case TSLB_STORAGE_FP16: return fn(_Float16{});
+    case TSLB_STORAGE_BF16: return fn(bf16_t{});
+    default: return fn(float{});
+  }
+}
+
+template <typename S>
+S *field_ptr(void *base, size_t fs, int q) {
+  return (S *)base + (size_t)q * fs;
+}
+template <typename S>
+FieldsT<S> fields_of(void *base, size_t fs) {
+  FieldsT<S> f;
+  for (int q = 0; q < TSLB_NFIELDS; q++) f.f[q] = field_ptr<S>(base, fs, q);
   return f;
 }
-CFields cfields_of(const float *base, size_t fs) {
-  CFields f;
-  for (int q = 0; q < TSLB_NFIELDS; q++) f.f[q] = base + (size_t)q * fs;
+template <typename S>
+CFieldsT<S> cfields_of(const void *base, size_t fs) {
+  CFieldsT<S> f;
+  for (int q = 0; q < TSLB_NFIELDS; q++) f.f[q] = (const S *)base + (size_t)q * fs;
   return f;
 }
 
@@ -1025,7 +1134,7 @@ hipEvent_t take_event(tslb_ctx *c) {
     e = c->ev_pool.back();
     c->ev_pool.pop_back();
   } else {
-    hipEventCreate(&e);
+    (void)hipEventCreate(&e);
   }
   return e;
 }
@@ -1046,9 +1155,15 @@ int drain_timing(tslb_ctx *c) {
   return TSLB_OK;
 }
 
+// zero a field range of the state buffer (any storage type: all-zero bits)
+void zero_fields(tslb_ctx *c, void *base, int q0, int q1) {
+  const size_t words = (size_t)(q1 - q0) * c->fs * c->es / 4;
+  float *start = (float *)((char *)base + (size_t)q0 * c->fs * c->es);
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, c->stream, start, 0.f,
+                     (int)words);
+}
+
 int launch_normals(tslb_ctx *c) {
-  const float *base = c->buf[c->cur];
-  const int nb = (int)((c->n + 255) / 256);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = take_event(c);
@@ -1056,9 +1171,13 @@ int launch_normals(tslb_ctx *c) {
     HIPCK(hipEventRecord(e0, c->stream));
   }
   const int nblk = c->kp.tiles_x * c->kp.tiles_y * c->kp.nzc;
-  hipLaunchKernelGGL(k_normals_tiled, dim3(nblk), dim3(NTHR), 0, c->stream, base + TSLB_PHI * c->fs,
-                     c->nrm, c->nrm + c->fs, c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->links,
-                     c->has_mask, c->kp);
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_normals_tiled<S>, dim3(nblk), dim3(NTHR), 0, c->stream,
+                       field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
+                       c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->links, c->has_mask, c->kp);
+    return 0;
+  });
   HIPCK(hipGetLastError());
   if (c->timing) {
     HIPCK(hipEventRecord(e1, c->stream));
@@ -1069,12 +1188,8 @@ int launch_normals(tslb_ctx *c) {
 }
 
 int launch_fused(tslb_ctx *c) {
-  const float *src = c->buf[c->cur];
-  float *dst = c->buf[1 - c->cur];
   const Params &p = c->kp;
   const int nblk = p.tiles_x * p.tiles_y * p.nzc;
-  CFields s = cfields_of(src, c->fs);
-  Fields d = fields_of(dst, c->fs);
   const float *n0 = c->nrm, *n1 = c->nrm + c->fs, *n2 = c->nrm + 2 * c->fs, *ka = c->nrm + 3 * c->fs;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
@@ -1086,28 +1201,40 @@ int launch_fused(tslb_ctx *c) {
                    (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
   const bool pf = (c->variant & 1) != 0;
   const bool w4 = (c->variant & 2) != 0;
-#define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                  \
-  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    CFieldsT<S> s = cfields_of<S>(c->buf[c->cur], c->fs);
+    FieldsT<S> d = fields_of<S>(c->buf[1 - c->cur], c->fs);
+#define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                          \
+  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
                      n0, n1, n2, ka, c->links, p)
-#define TSLB_LAUNCH_V(NC, MASK, ADD)                 \
-  do {                                               \
-    if (pf && w4) TSLB_LAUNCH(NC, MASK, ADD, true, 4);   \
-    else if (pf) TSLB_LAUNCH(NC, MASK, ADD, true, 1);    \
-    else if (w4) TSLB_LAUNCH(NC, MASK, ADD, false, 4);   \
-    else TSLB_LAUNCH(NC, MASK, ADD, false, 1);           \
+    // FP32 storage keeps the four kernel variants (prefetch x register cap)
+    // selectable for sweeps; reduced storage uses prefetch with the cap
+#define TSLB_LAUNCH_V(NC, MASK, ADD)                                      \
+  do {                                                                    \
+    if constexpr (sizeof(S) == 4) {                                       \
+      if (pf && w4) TSLB_LAUNCH(NC, MASK, ADD, true, 4);                  \
+      else if (pf) TSLB_LAUNCH(NC, MASK, ADD, true, 1);                   \
+      else if (w4) TSLB_LAUNCH(NC, MASK, ADD, false, 4);                  \
+      else TSLB_LAUNCH(NC, MASK, ADD, false, 1);                          \
+    } else {                                                              \
+      TSLB_LAUNCH(NC, MASK, ADD, true, 4);                                \
+    }                                                                     \
   } while (0)
-  if (p.ncomp == 2) {
-    if (c->has_mask) TSLB_LAUNCH_V(2, true, false);
-    else TSLB_LAUNCH_V(2, false, false);
-  } else if (add) {
-    if (c->has_mask) TSLB_LAUNCH_V(1, true, true);
-    else TSLB_LAUNCH_V(1, false, true);
-  } else {
-    if (c->has_mask) TSLB_LAUNCH_V(1, true, false);
-    else TSLB_LAUNCH_V(1, false, false);
-  }
+    if (p.ncomp == 2) {
+      if (c->has_mask) TSLB_LAUNCH_V(2, true, false);
+      else TSLB_LAUNCH_V(2, false, false);
+    } else if (add) {
+      if (c->has_mask) TSLB_LAUNCH_V(1, true, true);
+      else TSLB_LAUNCH_V(1, false, true);
+    } else {
+      if (c->has_mask) TSLB_LAUNCH_V(1, true, false);
+      else TSLB_LAUNCH_V(1, false, false);
+    }
 #undef TSLB_LAUNCH_V
 #undef TSLB_LAUNCH
+    return 0;
+  });
   HIPCK(hipGetLastError());
   if (c->timing) {
     HIPCK(hipEventRecord(e1, c->stream));
@@ -1119,7 +1246,8 @@ int launch_fused(tslb_ctx *c) {
   }
   if (p.ncomp == 1) {
     // phi is carried unchanged by one-component runs
-    HIPCK(hipMemcpyAsync(dst + TSLB_PHI * c->fs, src + TSLB_PHI * c->fs, c->n * sizeof(float),
+    HIPCK(hipMemcpyAsync((char *)c->buf[1 - c->cur] + TSLB_PHI * c->fs * c->es,
+                         (char *)c->buf[c->cur] + TSLB_PHI * c->fs * c->es, c->n * c->es,
                          hipMemcpyDeviceToDevice, c->stream));
   }
   c->cur = 1 - c->cur;
@@ -1130,9 +1258,13 @@ int launch_fused(tslb_ctx *c) {
 int launch_zbc(tslb_ctx *c) {
   if (!c->zbc) return TSLB_OK;
   const int plane = c->prm.nx * c->prm.ny;
-  hipLaunchKernelGGL(k_zplane_bc, dim3((plane + 255) / 256), dim3(256), 0, c->stream,
-                     fields_of(c->buf[c->cur], c->fs), c->w_in, c->zbc_phi ? c->phi_in : nullptr,
-                     c->prm.ncomp, c->kp);
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_zplane_bc<S>, dim3((plane + 255) / 256), dim3(256), 0, c->stream,
+                       fields_of<S>(c->buf[c->cur], c->fs), c->w_in, c->zbc_phi ? c->phi_in : nullptr,
+                       c->prm.ncomp, c->kp);
+    return 0;
+  });
   HIPCK(hipGetLastError());
   return TSLB_OK;
 }
@@ -1168,8 +1300,9 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   if (n > 0x7fffffffLL) return fail(TSLB_E_ARG, "nx*ny*nz must fit in int32");
   if (p->ncomp != 1 && p->ncomp != 2) return fail(TSLB_E_ARG, "ncomp must be 1 or 2");
   if (!(p->tau > 0.5f)) return fail(TSLB_E_ARG, "tau must be > 0.5");
-  if (p->storage != TSLB_STORAGE_FP32)
-    return fail(TSLB_E_ARG, "only FP32 storage is available in this build");
+  if (p->storage != TSLB_STORAGE_FP32 && p->storage != TSLB_STORAGE_FP16 &&
+      p->storage != TSLB_STORAGE_BF16)
+    return fail(TSLB_E_ARG, "storage must be TSLB_STORAGE_FP32, _FP16 or _BF16");
   if (p->force_mode != TSLB_FORCE_ADDITIVE && p->force_mode != TSLB_FORCE_SHIFT)
     return fail(TSLB_E_ARG, "force_mode must be TSLB_FORCE_ADDITIVE or TSLB_FORCE_SHIFT");
   int ndev = 0;
@@ -1201,6 +1334,18 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.tiles_x = (p->nx + TX - 1) / TX;
   k.tiles_y = (p->ny + TY - 1) / TY;
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
+  // storage scales: FP16 keeps rho-1 (x64) and Pi^neq (x4096) in its normal
+  // range; BF16 has FP32's exponent range and FP32 needs none
+  c->es = p->storage == TSLB_STORAGE_FP32 ? 4 : 2;
+  for (int q = 0; q < TSLB_NFIELDS; q++) {
+    float sc = 1.f;
+    if (p->storage == TSLB_STORAGE_FP16) {
+      if (q == TSLB_RHO) sc = 64.f;
+      if (q >= TSLB_PXX && q <= TSLB_PYZ) sc = 4096.f;
+    }
+    k.sc[q] = sc;
+    k.isc[q] = 1.f / sc;
+  }
 
   if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 3;
   int rc = TSLB_OK;
@@ -1209,7 +1354,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     delete c;
     return fail(TSLB_E_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
   }
-  const size_t fb = c->fs * TSLB_NFIELDS * sizeof(float);
+  const size_t fb = c->fs * TSLB_NFIELDS * c->es;
   if ((rc = dev_alloc(c, (void **)&c->buf[0], fb)) || (rc = dev_alloc(c, (void **)&c->buf[1], fb)) ||
       (p->ncomp == 2 && (rc = dev_alloc(c, (void **)&c->nrm, 4 * c->fs * sizeof(float)))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
@@ -1219,10 +1364,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     return fail(rc, msg);
   }
   // zero state: rho-1 = 0, all moments 0
-  for (int b = 0; b < 2; b++) {
-    const int tot = (int)(c->fs * TSLB_NFIELDS);
-    hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->buf[b], 0.f, tot);
-  }
+  for (int b = 0; b < 2; b++) zero_fields(c, c->buf[b], 0, TSLB_NFIELDS);
   if (c->nrm) {
     const int tot = (int)(4 * c->fs);
     hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->nrm, 0.f, tot);
@@ -1238,24 +1380,24 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
 
 int tslb_destroy(tslb_ctx *c) {
   if (!c) return TSLB_OK;
-  hipSetDevice(c->prm.device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->prm.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int k = 0; k < 2; k++) {
     for (auto &pr : c->pending[k]) {
-      hipEventDestroy(pr.first);
-      hipEventDestroy(pr.second);
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
     }
   }
-  for (auto e : c->ev_pool) hipEventDestroy(e);
-  hipFree(c->buf[0]);
-  hipFree(c->buf[1]);
-  hipFree(c->nrm);
-  hipFree(c->links);
-  hipFree(c->part);
-  hipFree(c->red);
-  hipFree(c->w_in);
-  hipFree(c->phi_in);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  (void)hipFree(c->buf[0]);
+  (void)hipFree(c->buf[1]);
+  (void)hipFree(c->nrm);
+  (void)hipFree(c->links);
+  (void)hipFree(c->part);
+  (void)hipFree(c->red);
+  (void)hipFree(c->w_in);
+  (void)hipFree(c->phi_in);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TSLB_OK;
 }
@@ -1278,7 +1420,7 @@ int tslb_set_isfluid(tslb_ctx *c, const int32_t *isf) {
                        c->links, c->kp);
     e = hipStreamSynchronize(c->stream);
   }
-  hipFree(tmp);
+  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(TSLB_E_HIP, std::string("set_isfluid: ") + hipGetErrorString(e));
   c->has_mask = 1;
   return TSLB_OK;
@@ -1288,16 +1430,18 @@ int tslb_upload_field(tslb_ctx *c, int field, const float *data) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (field < 0 || field >= TSLB_NFIELDS || !data) return fail(TSLB_E_ARG, "bad field or pointer");
-  float *d = c->buf[c->cur] + (size_t)field * c->fs;
-  if (field == TSLB_RHO) {
-    std::vector<float> tmp(c->n);
-    for (size_t i = 0; i < c->n; i++) tmp[i] = data[i] - 1.0f;
-    HIPCK(hipMemcpyAsync(d, tmp.data(), c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCK(hipStreamSynchronize(c->stream));
-  } else {
-    HIPCK(hipMemcpyAsync(d, data, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    HIPCK(hipStreamSynchronize(c->stream));
-  }
+  // stage the host floats in the second buffer (scratch between steps), then
+  // convert into the state buffer (rho -> rho-1, storage type and scale)
+  float *tmp = (float *)c->buf[1 - c->cur];
+  HIPCK(hipMemcpyAsync(tmp, data, c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_to_storage<S>, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, tmp,
+                       field_ptr<S>(c->buf[c->cur], c->fs, field), (int)c->n, field, c->kp);
+    return 0;
+  });
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
   c->normals_fresh = 0;
   return TSLB_OK;
 }
@@ -1315,11 +1459,16 @@ int tslb_download_field(tslb_ctx *c, int field, float *data) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (field < 0 || field >= TSLB_NFIELDS || !data) return fail(TSLB_E_ARG, "bad field or pointer");
-  const float *d = c->buf[c->cur] + (size_t)field * c->fs;
-  HIPCK(hipMemcpyAsync(data, d, c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  float *tmp = (float *)c->buf[1 - c->cur];
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_from_storage<S>, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
+                       (const S *)field_ptr<S>(c->buf[c->cur], c->fs, field), tmp, (int)c->n, field, c->kp);
+    return 0;
+  });
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(data, tmp, c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
-  if (field == TSLB_RHO)
-    for (size_t i = 0; i < c->n; i++) data[i] = 1.0f + data[i];
   return TSLB_OK;
 }
 
@@ -1348,18 +1497,22 @@ int tslb_download_normals(tslb_ctx *c, float *n3) {
 int tslb_init_equilibrium(tslb_ctx *c, int prepush) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  float *base = c->buf[c->cur];
+  void *base = c->buf[c->cur];
   const int nb = (int)((c->n + 255) / 256);
-  for (int q = TSLB_PXX; q <= TSLB_PYZ; q++)
-    hipLaunchKernelGGL(k_fill, dim3(nb), dim3(256), 0, c->stream, base + q * c->fs, 0.f, (int)c->n);
+  zero_fields(c, base, TSLB_PXX, TSLB_PYZ + 1);
   if (c->prm.ncomp == 2 && prepush) {
-    float *tmp = c->buf[1 - c->cur] + TSLB_PHI * c->fs;
-    hipLaunchKernelGGL(k_prepush, dim3(nb), dim3(256), 0, c->stream, base + TSLB_PHI * c->fs,
-                       base + TSLB_U * c->fs, base + TSLB_V * c->fs, base + TSLB_W * c->fs, tmp,
-                       c->links, c->has_mask, c->kp);
+    with_storage(c, [&](auto tag) {
+      using S = decltype(tag);
+      S *tmp = field_ptr<S>(c->buf[1 - c->cur], c->fs, TSLB_PHI);
+      hipLaunchKernelGGL(k_prepush<S>, dim3(nb), dim3(256), 0, c->stream, field_ptr<S>(base, c->fs, TSLB_PHI),
+                         field_ptr<S>(base, c->fs, TSLB_U), field_ptr<S>(base, c->fs, TSLB_V),
+                         field_ptr<S>(base, c->fs, TSLB_W), tmp, c->links, c->has_mask, c->kp);
+      return 0;
+    });
     HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(base + TSLB_PHI * c->fs, tmp, c->n * sizeof(float), hipMemcpyDeviceToDevice,
-                         c->stream));
+    HIPCK(hipMemcpyAsync((char *)base + TSLB_PHI * c->fs * c->es,
+                         (char *)c->buf[1 - c->cur] + TSLB_PHI * c->fs * c->es, c->n * c->es,
+                         hipMemcpyDeviceToDevice, c->stream));
   }
   HIPCK(hipStreamSynchronize(c->stream));
   c->normals_fresh = 0;
@@ -1420,8 +1573,12 @@ int tslb_seed_noise(tslb_ctx *c, int field, float amp, int l) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (field < 0 || field >= TSLB_NFIELDS) return fail(TSLB_E_ARG, "bad field");
-  float *d = c->buf[c->cur] + (size_t)field * c->fs;
-  hipLaunchKernelGGL(k_seed_noise, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, d, amp, l, c->kp);
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_seed_noise<S>, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream,
+                       field_ptr<S>(c->buf[c->cur], c->fs, field), field, amp, l, c->kp);
+    return 0;
+  });
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(c->stream));
   c->normals_fresh = 0;
@@ -1433,8 +1590,12 @@ int tslb_reduce(tslb_ctx *c, double out[7]) {
   if (rc) return rc;
   if (!out) return fail(TSLB_E_ARG, "null pointer");
   const int nblk = (int)((c->n + 255) / 256) < kRedBlocks ? (int)((c->n + 255) / 256) : kRedBlocks;
-  hipLaunchKernelGGL(k_reduce_part, dim3(nblk), dim3(256), 0, c->stream, cfields_of(c->buf[c->cur], c->fs),
-                     c->links, c->has_mask, c->kp, c->part);
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_reduce_part<S>, dim3(nblk), dim3(256), 0, c->stream,
+                       cfields_of<S>(c->buf[c->cur], c->fs), c->links, c->has_mask, c->kp, c->part);
+    return 0;
+  });
   HIPCK(hipGetLastError());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, c->part, nblk, c->red);
   HIPCK(hipGetLastError());
@@ -1501,13 +1662,13 @@ int tslb_stream_copy(tslb_ctx *c, double *gbs) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!gbs) return fail(TSLB_E_ARG, "null pointer");
-  const long long n4 = (long long)(c->fs * TSLB_NFIELDS / 4);
+  const long long n4 = (long long)(c->fs * TSLB_NFIELDS * c->es / 16);
   hipEvent_t e0 = take_event(c), e1 = take_event(c);
   float best = 1e30f;
   for (int rep = 0; rep < 4; rep++) {
     HIPCK(hipEventRecord(e0, c->stream));
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, c->stream, (const float4 *)c->buf[c->cur],
-                       (float4 *)c->buf[1 - c->cur], n4);
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, c->stream, (const v4f *)c->buf[c->cur],
+                       (v4f *)c->buf[1 - c->cur], n4);
     HIPCK(hipGetLastError());
     HIPCK(hipEventRecord(e1, c->stream));
     HIPCK(hipEventSynchronize(e1));
