@@ -180,37 +180,12 @@ __device__ __forceinline__ float cm(float a) {
   else if constexpr (C > 0) return a;
   else return -a;
 }
-// c . (a, b, c) with only the non-zero components
-template <int X, int Y, int Z>
-__device__ __forceinline__ float cdot(float a, float b, float c) {
-  if constexpr (X != 0 && Y != 0 && Z != 0) return cm<X>(a) + cm<Y>(b) + cm<Z>(c);
-  else if constexpr (X != 0 && Y != 0) return cm<X>(a) + cm<Y>(b);
-  else if constexpr (X != 0 && Z != 0) return cm<X>(a) + cm<Z>(c);
-  else if constexpr (Y != 0 && Z != 0) return cm<Y>(b) + cm<Z>(c);
-  else if constexpr (X != 0) return cm<X>(a);
-  else if constexpr (Y != 0) return cm<Y>(b);
-  else if constexpr (Z != 0) return cm<Z>(c);
-  else return 0.f;
-}
 
 // ---------------------------------------------------------------------------
 // Post-collision populations of one node, shifted by the weights:
 //   h_l = f*_l - w_l,  f*_l = feq_l(rho,u_eq) + (1-omega) w_l a^neq_l(Pi,u) [+F]
-// (recursive_2c_TSLB3D.f90:637-806, recursive_TSLB3D.f90:556-707).  Evaluated
-// per pair (l, opp(l)) as w_l (E +- O) with E, O the even and odd parts in c:
-//   E/w = drho - 3/2 rho u_eq^2 - Ts/3 + 9/2 rho (c.u_eq)^2 + c.Ps.c
-//   O/w = c.u_eq (3 rho - 9/2 rho u_eq^2 + 9/2 rho (c.u_eq)^2)
-//         + c.u (3 c.Ps.c - Ts) - 2 c.(Ps u) + 3 c.F
-// with Ps = 9/2 (1-omega) Pi^neq and Ts = tr Ps.
+// (recursive_2c_TSLB3D.f90:637-806, recursive_TSLB3D.f90:556-707).
 // ---------------------------------------------------------------------------
-struct NodeC {
-  float K0, A, C1, Ts;
-  float ux, uy, uz, sx, sy, sz;  // u and u_eq
-  float pxx, pyy, pzz, pxy2, pxz2, pyz2;  // Ps, off-diagonals doubled
-  float qx, qy, qz;                       // Ps u
-  float f3x, f3y, f3z;                    // 3 F (additive forcing)
-};
-
 // Destination of the populations of one source: directions that move in the
 // x-y plane go to the LDS slot of the source, 0, 5 and 6 stay in registers.
 struct PopSink {
@@ -226,92 +201,108 @@ struct PopSink {
   }
 };
 
-template <int L, bool SHIFT, bool ADD, typename Sink>
-__device__ __forceinline__ void pair_pop(const NodeC &n, Sink &h) {
-  constexpr int X = kCX[L], Y = kCY[L], Z = kCZ[L];
-  const float cus = cdot<X, Y, Z>(n.sx, n.sy, n.sz);
-  const float cu = SHIFT ? cdot<X, Y, Z>(n.ux, n.uy, n.uz) : cus;
-  // c.Ps.c
-  float cpc;
-  {
-    float d = 0.f;
-    bool any = false;
-    if constexpr (X != 0) { d = n.pxx; any = true; }
-    if constexpr (Y != 0) { d = any ? d + n.pyy : n.pyy; any = true; }
-    if constexpr (Z != 0) { d = any ? d + n.pzz : n.pzz; any = true; }
-    if constexpr (X != 0 && Y != 0) d = d + cm<X * Y>(n.pxy2);
-    if constexpr (X != 0 && Z != 0) d = d + cm<X * Z>(n.pxz2);
-    if constexpr (Y != 0 && Z != 0) d = d + cm<Y * Z>(n.pyz2);
-    cpc = d;
+// Shifted post-collision populations by the D3Q27 tensor-product transform.
+// Raw moments M_abc = sum_l cx^a cy^b cz^c h_l (a,b,c in {0,1,2}) of
+// h = f* - w follow from the Hermite coefficients of f*: a0 = rho,
+// a1 = rho V (+F), a2 = rho VV + (1-omega) Pi, a3 = rho VVV + (1-omega) sym(u Pi)
+// on the seven D3Q27 third-order components, zero above; with cs^2 = 1/3:
+//   M_2.. = a2_xx + cs^2 a0 (and products of such factors per axis).
+// Each M_abc is stored pre-multiplied by 2^-(number of nonzero exponents) so
+// that the 1D inverse (m0, m1, m2) -> f(-1) = m2 - m1, f(0) = m0 - 2 m2,
+// f(+1) = m2 + m1 costs three operations per triple.
+template <bool ADD, typename Sink>
+__device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, float Vz,
+                                            const Params &p, Sink &h) {
+  const float drho = m[0], rho = 1.f + drho;
+  const float ux = m[1], uy = m[2], uz = m[3];
+  const float om1 = p.om1;
+  const float Bxx = om1 * m[4], Byy = om1 * m[5], Bzz = om1 * m[6];
+  const float Bxy = om1 * m[7], Bxz = om1 * m[8], Byz = om1 * m[9];
+  const float rVx = rho * Vx, rVy = rho * Vy, rVz = rho * Vz;
+  float a1x = rVx, a1y = rVy, a1z = rVz;
+  if constexpr (ADD) {
+    a1x += p.fx;
+    a1y += p.fy;
+    a1z += p.fz;
   }
-  const float cq = cdot<X, Y, Z>(n.qx, n.qy, n.qz);
-  const float cus2 = cus * cus;
-  const float E = fmaf(n.A, cus2, n.K0) + cpc;
-  float O = cus * fmaf(n.A, cus2, n.C1);
-  O = fmaf(cu, fmaf(3.f, cpc, -n.Ts), O);
-  O = fmaf(-2.f, cq, O);
-  if constexpr (ADD) O = O + cdot<X, Y, Z>(n.f3x, n.f3y, n.f3z);
-  constexpr float w = wgt(L);
-  h.template put<L>(w * (E + O));
-  h.template put<kOPP[L]>(w * (E - O));
+  const float rVxx = rVx * Vx, rVyy = rVy * Vy, rVzz = rVz * Vz;
+  const float rVxy = rVx * Vy, rVxz = rVx * Vz, rVyz = rVy * Vz;
+  const float a2xx = rVxx + Bxx, a2yy = rVyy + Byy, a2zz = rVzz + Bzz;
+  const float a2xy = rVxy + Bxy, a2xz = rVxz + Bxz, a2yz = rVyz + Byz;
+  const float a3xxy = fmaf(rVxx, Vy, fmaf(uy, Bxx, 2.f * ux * Bxy));
+  const float a3xxz = fmaf(rVxx, Vz, fmaf(uz, Bxx, 2.f * ux * Bxz));
+  const float a3xyy = fmaf(rVyy, Vx, fmaf(ux, Byy, 2.f * uy * Bxy));
+  const float a3yyz = fmaf(rVyy, Vz, fmaf(uz, Byy, 2.f * uy * Byz));
+  const float a3xzz = fmaf(rVzz, Vx, fmaf(ux, Bzz, 2.f * uz * Bxz));
+  const float a3yzz = fmaf(rVzz, Vy, fmaf(uy, Bzz, 2.f * uz * Byz));
+  const float a3xyz = fmaf(rVxy, Vz, fmaf(ux, Byz, fmaf(uy, Bxz, uz * Bxy)));
+  constexpr float c3 = 1.f / 3.f, c9 = 1.f / 9.f, c27 = 1.f / 27.f;
+  // M[a][b][c], scaled by 2^-(#nonzero)
+  float M[3][3][3];
+  M[0][0][0] = drho;
+  M[1][0][0] = 0.5f * a1x;
+  M[0][1][0] = 0.5f * a1y;
+  M[0][0][1] = 0.5f * a1z;
+  M[2][0][0] = 0.5f * fmaf(c3, drho, a2xx);
+  M[0][2][0] = 0.5f * fmaf(c3, drho, a2yy);
+  M[0][0][2] = 0.5f * fmaf(c3, drho, a2zz);
+  M[1][1][0] = 0.25f * a2xy;
+  M[1][0][1] = 0.25f * a2xz;
+  M[0][1][1] = 0.25f * a2yz;
+  M[2][1][0] = 0.25f * fmaf(c3, a1y, a3xxy);
+  M[2][0][1] = 0.25f * fmaf(c3, a1z, a3xxz);
+  M[1][2][0] = 0.25f * fmaf(c3, a1x, a3xyy);
+  M[0][2][1] = 0.25f * fmaf(c3, a1z, a3yyz);
+  M[1][0][2] = 0.25f * fmaf(c3, a1x, a3xzz);
+  M[0][1][2] = 0.25f * fmaf(c3, a1y, a3yzz);
+  M[1][1][1] = 0.125f * a3xyz;
+  M[2][2][0] = 0.25f * fmaf(c9, drho, c3 * (a2xx + a2yy));
+  M[2][0][2] = 0.25f * fmaf(c9, drho, c3 * (a2xx + a2zz));
+  M[0][2][2] = 0.25f * fmaf(c9, drho, c3 * (a2yy + a2zz));
+  M[2][1][1] = (0.125f * c3) * a2yz;
+  M[1][2][1] = (0.125f * c3) * a2xz;
+  M[1][1][2] = (0.125f * c3) * a2xy;
+  M[2][2][1] = 0.125f * fmaf(c9, a1z, c3 * (a3xxz + a3yyz));
+  M[2][1][2] = 0.125f * fmaf(c9, a1y, c3 * (a3xxy + a3yzz));
+  M[1][2][2] = 0.125f * fmaf(c9, a1x, c3 * (a3xyy + a3xzz));
+  M[2][2][2] = 0.125f * fmaf(c27, drho, c9 * (a2xx + a2yy + a2zz));
+  // pass z: G[a][b][cz]
+  float G[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      const float m0 = M[a][b][0], m1 = M[a][b][1], m2 = M[a][b][2];
+      G[a][b][0] = m2 - m1;
+      G[a][b][1] = fmaf(-2.f, m2, m0);
+      G[a][b][2] = m2 + m1;
+    }
+  // pass y: K[a][cy][cz]
+  float K[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int cz = 0; cz < 3; cz++) {
+      const float m0 = G[a][0][cz], m1 = G[a][1][cz], m2 = G[a][2][cz];
+      K[a][0][cz] = m2 - m1;
+      K[a][1][cz] = fmaf(-2.f, m2, m0);
+      K[a][2][cz] = m2 + m1;
+    }
+  // pass x and store
+#define MT_ROW(CY, CZ)                                              \
+  {                                                                 \
+    const float m0 = K[0][CY + 1][CZ + 1], m1 = K[1][CY + 1][CZ + 1]; \
+    const float m2 = K[2][CY + 1][CZ + 1];                           \
+    h.template put<dir_of(-1, CY, CZ)>(m2 - m1);                     \
+    h.template put<dir_of(0, CY, CZ)>(fmaf(-2.f, m2, m0));          \
+    h.template put<dir_of(1, CY, CZ)>(m2 + m1);                      \
+  }
+  MT_ROW(-1, -1) MT_ROW(-1, 0) MT_ROW(-1, 1)
+  MT_ROW(0, -1) MT_ROW(0, 0) MT_ROW(0, 1)
+  MT_ROW(1, -1) MT_ROW(1, 0) MT_ROW(1, 1)
+#undef MT_ROW
 }
 
-template <bool SHIFT, bool ADD, typename Sink>
-__device__ __forceinline__ void all_pops(const NodeC &n, Sink &h) {
-  h.template put<0>(wgt(0) * n.K0);
-  pair_pop<1, SHIFT, ADD>(n, h);
-  pair_pop<3, SHIFT, ADD>(n, h);
-  pair_pop<5, SHIFT, ADD>(n, h);
-  pair_pop<7, SHIFT, ADD>(n, h);
-  pair_pop<9, SHIFT, ADD>(n, h);
-  pair_pop<11, SHIFT, ADD>(n, h);
-  pair_pop<13, SHIFT, ADD>(n, h);
-  pair_pop<15, SHIFT, ADD>(n, h);
-  pair_pop<17, SHIFT, ADD>(n, h);
-  pair_pop<19, SHIFT, ADD>(n, h);
-  pair_pop<21, SHIFT, ADD>(n, h);
-  pair_pop<23, SHIFT, ADD>(n, h);
-  pair_pop<25, SHIFT, ADD>(n, h);
-}
-
-// Per-node constants.  m = {drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz}; (esx,esy,esz)
-// is the velocity shift (CSF + body force + linear forcing) when SHIFT.
-__device__ __forceinline__ NodeC node_consts(const float *m, float esx, float esy, float esz,
-                                             const Params &p, bool shift, bool add) {
-  NodeC n;
-  const float drho = m[0];
-  const float rho = 1.f + drho;
-  n.ux = m[1];
-  n.uy = m[2];
-  n.uz = m[3];
-  n.sx = shift ? n.ux + esx : n.ux;
-  n.sy = shift ? n.uy + esy : n.uy;
-  n.sz = shift ? n.uz + esz : n.uz;
-  const float uu = n.sx * n.sx + n.sy * n.sy + n.sz * n.sz;
-  const float B = 4.5f * p.om1;
-  n.pxx = B * m[4];
-  n.pyy = B * m[5];
-  n.pzz = B * m[6];
-  const float pxy = B * m[7], pxz = B * m[8], pyz = B * m[9];
-  n.pxy2 = 2.f * pxy;
-  n.pxz2 = 2.f * pxz;
-  n.pyz2 = 2.f * pyz;
-  n.Ts = n.pxx + n.pyy + n.pzz;
-  n.A = 4.5f * rho;
-  n.K0 = fmaf(-1.5f * rho, uu, drho) - n.Ts * (1.f / 3.f);
-  n.C1 = fmaf(-n.A, uu, 3.f * rho);
-  n.qx = n.pxx * n.ux + pxy * n.uy + pxz * n.uz;
-  n.qy = pxy * n.ux + n.pyy * n.uy + pyz * n.uz;
-  n.qz = pxz * n.ux + pyz * n.uy + n.pzz * n.uz;
-  if (add) {
-    n.f3x = 3.f * p.fx;
-    n.f3y = 3.f * p.fy;
-    n.f3z = 3.f * p.fz;
-  } else {
-    n.f3x = n.f3y = n.f3z = 0.f;
-  }
-  return n;
-}
 
 // ---------------------------------------------------------------------------
 // Colour-gradient stencil, grouped as the reference (recursive_2c_TSLB3D.f90:
@@ -379,8 +370,7 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, PopS
     const float esx = sx + p.fx + p.lin_a * m[1];
     const float esy = sy + p.fy + p.lin_a * m[2];
     const float esz = sz + p.fz + p.lin_a * m[3];
-    const NodeC nc = node_consts(m, esx, esy, esz, p, true, false);
-    all_pops<true, false>(nc, h);
+    all_pops_mt<false>(m, m[1] + esx, m[2] + esy, m[3] + esz, p, h);
     // D3Q7 phase field with sharpening, recursive_2c_TSLB3D.f90:808-831
     const float p1g = 0.125f;
     const float sh = p1g * p.sharp_c * phi * (1.f - phi);
@@ -394,12 +384,10 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, PopS
     gq[6] = base * (1.f - 3.f * m[3]) - sh * n2;
   } else {
     if (p.shift) {
-      const NodeC nc = node_consts(m, p.fx + p.lin_a * m[1], p.fy + p.lin_a * m[2],
-                                   p.fz + p.lin_a * m[3], p, true, false);
-      all_pops<true, false>(nc, h);
+      all_pops_mt<false>(m, m[1] + (p.fx + p.lin_a * m[1]), m[2] + (p.fy + p.lin_a * m[2]),
+                         m[3] + (p.fz + p.lin_a * m[3]), p, h);
     } else {
-      const NodeC nc = node_consts(m, 0.f, 0.f, 0.f, p, false, ADD);
-      all_pops<false, ADD>(nc, h);
+      all_pops_mt<ADD>(m, m[1], m[2], m[3], p, h);
     }
   }
 }
