@@ -344,16 +344,20 @@ template <int NC, typename S>
 __device__ __forceinline__ void load_src(const CFieldsT<S> &src, const float *__restrict__ n0a,
                                          const float *__restrict__ n1a,
                                          const float *__restrict__ n2a,
-                                         const float *__restrict__ ka, int id, const Params &p,
-                                         Src<NC> &s) {
+                                         const float *__restrict__ ka, int id, bool use_n,
+                                         const Params &p, Src<NC> &s) {
 #pragma unroll
   for (int q = 0; q < 10; q++) s.m[q] = ldf(src.f[q], id, p, q);
   if constexpr (NC == 2) {
     s.m[10] = ldf(src.f[TSLB_PHI], id, p, TSLB_PHI);
-    s.nn[0] = n0a[id];
-    s.nn[1] = n1a[id];
-    s.nn[2] = n2a[id];
-    s.kap = ka[id];
+    if (use_n) {
+      s.nn[0] = n0a[id];
+      s.nn[1] = n1a[id];
+      s.nn[2] = n2a[id];
+      s.kap = ka[id];
+    } else {
+      s.nn[0] = s.nn[1] = s.nn[2] = s.kap = 0.f;
+    }
   }
 }
 
@@ -479,6 +483,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
                                                 const float *__restrict__ n1a,
                                                 const float *__restrict__ n2a,
                                                 const float *__restrict__ ka,
+                                                const uint8_t *__restrict__ flags,
                                                 const uint32_t *__restrict__ links, Params p) {
   constexpr int NQ = NC == 2 ? 28 : 24;
   __shared__ float lds[NQ][NSLOT];
@@ -540,17 +545,22 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   auto fetch = [&](int zs, Src<NC> &o, bool &ook, Src<NC> &r, bool &rok) {
     int gz = 0;
     const bool zok = src_coord(zs, p.nz, p.pz, gz);
+    const int fz = gz * p.tiles_x * p.tiles_y;
     ook = zok && oxy;
     if (ook) {
       const int id = ox + p.nx * oy + plane * gz;
       if constexpr (MASK) ook = (links[id] & kSolid) == 0u;
-      if (ook) load_src<NC>(src, n0a, n1a, n2a, ka, id, p, o);
+      bool un = false;
+      if constexpr (NC == 2) un = flags[fz + (oy / TY) * p.tiles_x + ox / TX] != 0;
+      if (ook) load_src<NC>(src, n0a, n1a, n2a, ka, id, un, p, o);
     }
     rok = zok && rxy;
     if (rok) {
       const int id = rx + p.nx * ry + plane * gz;
       if constexpr (MASK) rok = (links[id] & kSolid) == 0u;
-      if (rok) load_src<NC>(src, n0a, n1a, n2a, ka, id, p, r);
+      bool un = false;
+      if constexpr (NC == 2) un = flags[fz + (ry / TY) * p.tiles_x + rx / TX] != 0;
+      if (rok) load_src<NC>(src, n0a, n1a, n2a, ka, id, un, p, r);
     }
   };
   if constexpr (PF) fetch(z0 - 1, so, so_ok, sr, sr_ok);
@@ -676,6 +686,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 // interface gate phi(1-phi)|n|^2 > 0.2 (0 elsewhere, :614-631).  Across a
 // non-periodic face phi is read at the nearest interior node and n is 0; n is
 // 0 on solid nodes.
+//
+// Sparse output: n and kappa are zero wherever phi is locally uniform (the
+// bulk of both phases), so each (tile, plane) gets a flag byte, 1 when any of
+// its n or kappa is nonzero; n and kappa are written only for flagged
+// tile-planes, and the fused pass reads them only there.  A plane whose phi
+// window z-1..z+1 is uniform skips the stencil.  `dense` writes everything.
 constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ phi,
@@ -683,8 +699,9 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
                                                          float *__restrict__ n1,
                                                          float *__restrict__ n2,
                                                          float *__restrict__ kap,
+                                                         uint8_t *__restrict__ flags,
                                                          const uint32_t *__restrict__ links,
-                                                         int has_mask, Params p) {
+                                                         int has_mask, int dense, Params p) {
   __shared__ float ph[5][NPSLOT];
   __shared__ float nl[3][3][NSLOT];
   const int tid = threadIdx.x;
@@ -700,6 +717,7 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
   const int x = x0 + tx, y = y0 + ty;
   const bool active = x < p.nx && y < p.ny;
   const int plane = p.nx * p.ny;
+  const int tiles_plane = p.tiles_x * p.tiles_y;
   auto clampw = [](int g, int n, int per) -> int {
     if (g >= 0 && g < n) return g;
     return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
@@ -716,52 +734,75 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
       dstp[sl] = ldf(phi, gx + p.nx * gy + plane * zz, p, TSLB_PHI);
     }
   };
+  // block vote (after a barrier): is the loaded phi plane z one value?
+  auto uniform_plane = [&](int z) -> bool {
+    const float *pl = ph[mod5(z)];
+    const float v0 = pl[0];
+    int diff = 0;
+    for (int sl = tid; sl < NPSLOT; sl += NTHR) diff |= (pl[sl] != v0);
+    return __syncthreads_or(diff) == 0;
+  };
   // normals of plane z on the (TX+2)x(TY+2) footprint, from phi planes z-1..z+1
-  auto normals_plane = [&](int z) {
+  auto normals_plane = [&](int z, bool zero) {
     int gz;
     const bool zin = src_coord(z, p.nz, p.pz, gz);
+    const int zs3 = mod3(z);
     for (int sl = tid; sl < NSLOT; sl += NTHR) {
-      const int sx = sl % SW, sy = sl / SW;
-      int gx, gy;
-      bool ok = zin && src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
-      if (ok && has_mask) ok = (links[gx + p.nx * gy + plane * gz] & kSolid) == 0u;
       float a = 0.f, bb = 0.f, c = 0.f;
-      if (ok) {
-        const float *pl[3] = {ph[mod5(z - 1)], ph[mod5(z)], ph[mod5(z + 1)]};
-        auto get = [&](int dx, int dy, int dz) -> float {
-          return pl[dz + 1][(sy + 1 + dy) * PW + (sx + 1 + dx)];
-        };
-        const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
-        const float den = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f;
-        a = gx_ / den;
-        bb = gy_ / den;
-        c = gz_ / den;
+      if (!zero) {
+        const int sx = sl % SW, sy = sl / SW;
+        int gx, gy;
+        bool ok = zin && src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
+        if (ok && has_mask) ok = (links[gx + p.nx * gy + plane * gz] & kSolid) == 0u;
+        if (ok) {
+          const float *pl[3] = {ph[mod5(z - 1)], ph[mod5(z)], ph[mod5(z + 1)]};
+          auto get = [&](int dx, int dy, int dz) -> float {
+            return pl[dz + 1][(sy + 1 + dy) * PW + (sx + 1 + dx)];
+          };
+          const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
+          const float den = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f;
+          a = gx_ / den;
+          bb = gy_ / den;
+          c = gz_ / den;
+        }
       }
-      const int zs3 = mod3(z);
       nl[zs3][0][sl] = a;
       nl[zs3][1][sl] = bb;
       nl[zs3][2][sl] = c;
     }
   };
+  // uniformity (u*) and value (v*) of phi planes z-2 .. z+2 for the current z
+  bool um2, um1, u0, up1, up2;
+  float vm2, vm1, v0, vp1, vp2;
   for (int z = z0 - 2; z <= z0 + 1; z++) load_phi(z);
   __syncthreads();
-  normals_plane(z0 - 1);
+  um2 = uniform_plane(z0 - 2);
+  vm2 = ph[mod5(z0 - 2)][0];
+  um1 = uniform_plane(z0 - 1);
+  vm1 = ph[mod5(z0 - 1)][0];
+  u0 = uniform_plane(z0);
+  v0 = ph[mod5(z0)][0];
+  up1 = uniform_plane(z0 + 1);
+  vp1 = ph[mod5(z0 + 1)][0];
+  normals_plane(z0 - 1, um2 && um1 && u0 && vm2 == vm1 && vm1 == v0);
   for (int z = z0; z < z1; z++) {
     load_phi(z + 2);
     __syncthreads();
-    normals_plane(z + 1);
-    if (z == z0) normals_plane(z0);
+    up2 = uniform_plane(z + 2);
+    vp2 = ph[mod5(z + 2)][0];
+    normals_plane(z + 1, u0 && up1 && up2 && v0 == vp1 && vp1 == vp2);
+    const bool zero_z = um1 && u0 && up1 && vm1 == v0 && v0 == vp1;
+    if (z == z0) normals_plane(z0, zero_z);
     __syncthreads();
-    if (active) {
-      const int id = x + p.nx * y + plane * z;
+    int any = 0;
+    float a = 0.f, bb = 0.f, c = 0.f, curv = 0.f;
+    if (!zero_z && active) {
       const int own = (ty + 1) * SW + (tx + 1);
       const int zm = mod3(z - 1), z0s = mod3(z), zp = mod3(z + 1);
-      const float a = nl[z0s][0][own], bb = nl[z0s][1][own], c = nl[z0s][2][own];
-      n0[id] = a;
-      n1[id] = bb;
-      n2[id] = c;
+      a = nl[z0s][0][own];
+      bb = nl[z0s][1][own];
+      c = nl[z0s][2][own];
       const float f = ph[mod5(z)][(ty + 2) * PW + (tx + 2)];
-      float curv = 0.f;
       const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
       if (ind > 0.2f) {
         auto nat = [&](int comp) {
@@ -775,8 +816,26 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
         const float dzz = grad_axis<2>(nat(2));
         curv = -(dxx + dyy + dzz);
       }
+      any = (a != 0.f) | (bb != 0.f) | (c != 0.f) | (curv != 0.f);
+    }
+    // block-uniform: skip the vote for a uniform window
+    const int flag = zero_z ? 0 : __syncthreads_or(any);
+    if ((flag || dense) && active) {
+      const int id = x + p.nx * y + plane * z;
+      n0[id] = a;
+      n1[id] = bb;
+      n2[id] = c;
       kap[id] = curv;
     }
+    if (tid == 0) flags[z * tiles_plane + tile_y * p.tiles_x + tile_x] = (uint8_t)(flag || dense);
+    um2 = um1;
+    vm2 = vm1;
+    um1 = u0;
+    vm1 = v0;
+    u0 = up1;
+    v0 = vp1;
+    up1 = up2;
+    vp1 = vp2;
   }
 }
 
@@ -1061,6 +1120,7 @@ struct tslb_ctx {
   void *buf[2] = {nullptr, nullptr};
   int cur = 0;
   float *nrm = nullptr;  // n0, n1, n2, kappa: 4 * fs floats
+  uint8_t *flags = nullptr;  // per (tile, plane): n / kappa nonzero there
   uint32_t *links = nullptr;
   int has_mask = 0;
   int normals_fresh = 0;
@@ -1151,7 +1211,7 @@ void zero_fields(tslb_ctx *c, void *base, int q0, int q1) {
                      (int)words);
 }
 
-int launch_normals(tslb_ctx *c) {
+int launch_normals(tslb_ctx *c, int dense = 0) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = take_event(c);
@@ -1163,7 +1223,8 @@ int launch_normals(tslb_ctx *c) {
     using S = decltype(tag);
     hipLaunchKernelGGL(k_normals_tiled<S>, dim3(nblk), dim3(NTHR), 0, c->stream,
                        field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
-                       c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->links, c->has_mask, c->kp);
+                       c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->flags, c->links, c->has_mask,
+                       dense, c->kp);
     return 0;
   });
   HIPCK(hipGetLastError());
@@ -1171,7 +1232,7 @@ int launch_normals(tslb_ctx *c) {
     HIPCK(hipEventRecord(e1, c->stream));
     c->pending[1].push_back({e0, e1});
   }
-  c->normals_fresh = 1;
+  c->normals_fresh = dense ? 2 : 1;
   return TSLB_OK;
 }
 
@@ -1195,7 +1256,7 @@ int launch_fused(tslb_ctx *c) {
     FieldsT<S> d = fields_of<S>(c->buf[1 - c->cur], c->fs);
 #define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                          \
   hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
-                     n0, n1, n2, ka, c->links, p)
+                     n0, n1, n2, ka, c->flags, c->links, p)
     // FP32 storage keeps the four kernel variants (prefetch x register cap)
     // selectable for sweeps; reduced storage uses prefetch with the cap
 #define TSLB_LAUNCH_V(NC, MASK, ADD)                                      \
@@ -1345,6 +1406,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   const size_t fb = c->fs * TSLB_NFIELDS * c->es;
   if ((rc = dev_alloc(c, (void **)&c->buf[0], fb)) || (rc = dev_alloc(c, (void **)&c->buf[1], fb)) ||
       (p->ncomp == 2 && (rc = dev_alloc(c, (void **)&c->nrm, 4 * c->fs * sizeof(float)))) ||
+      (p->ncomp == 2 &&
+       (rc = dev_alloc(c, (void **)&c->flags, (size_t)k.tiles_x * k.tiles_y * p->nz))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
       (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
     std::string msg = g_err;
@@ -1380,6 +1443,7 @@ int tslb_destroy(tslb_ctx *c) {
   (void)hipFree(c->buf[0]);
   (void)hipFree(c->buf[1]);
   (void)hipFree(c->nrm);
+  (void)hipFree(c->flags);
   (void)hipFree(c->links);
   (void)hipFree(c->part);
   (void)hipFree(c->red);
@@ -1474,7 +1538,8 @@ int tslb_download_normals(tslb_ctx *c, float *n3) {
   if (rc) return rc;
   if (!n3) return fail(TSLB_E_ARG, "null pointer");
   if (c->prm.ncomp != 2) return fail(TSLB_E_STATE, "normals exist only for two-component runs");
-  if (!c->normals_fresh && (rc = launch_normals(c))) return rc;
+  // the stepping pass writes n only where it is nonzero; recompute densely
+  if (c->normals_fresh != 2 && (rc = launch_normals(c, 1))) return rc;
   for (int a = 0; a < 3; a++)
     HIPCK(hipMemcpyAsync(n3 + (size_t)a * c->n, c->nrm + (size_t)a * c->fs, c->n * sizeof(float),
                          hipMemcpyDeviceToHost, c->stream));

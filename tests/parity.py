@@ -33,3 +33,14 @@ def field_errors(got, ref, mask=None):
 def worst(errs, fields=None):
     fields = fields or list(errs)
     return max(errs[f] for f in fields)
+
+
+def field_rms_errors(got, ref):
+    """dict field -> RMS of the difference over the RMS of the field's group."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    out = {}
+    for q, name in enumerate(FIELDS):
+        scale = max(float(np.sqrt(np.mean(ref[list(GROUP[q])] ** 2))), 1e-30)
+        out[name] = float(np.sqrt(np.mean((got[q] - ref[q]) ** 2))) / scale
+    return out
