@@ -98,6 +98,7 @@ struct Params {
   float sigma, sharp_c;
   float fx, fy, fz, lin_a;
   int z_chunk, tiles_x, tiles_y, nzc;
+  long long fs;             // element stride between the fields of a state buffer
   float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
   float isc[TSLB_NFIELDS];  // 1 / sc
 };
@@ -340,21 +341,78 @@ struct Src {
   float kap;                   // gated curvature (two components)
 };
 
+// Raw buffer access.  A descriptor (gfx9 layout: 32-bit data format, no
+// stride) covers one plane of one field, so a node is a loop-invariant 32-bit
+// byte offset from a scalar base, with no per-access 64-bit address
+// arithmetic; an access past the plane reads 0 and a store there is dropped.
+constexpr int kRsrcWord3 = 0x00020000;
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, kRsrcWord3);
+}
+template <typename S>
+__device__ __forceinline__ S bld(rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float bld<float>(rsrc_t r, uint32_t off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ _Float16 bld<_Float16>(rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ bf16_t bld<bf16_t>(rsrc_t r, uint32_t off) {
+  return bf16_t{__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)};
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t off, _Float16 v) {
+  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t off, bf16_t v) {
+  __builtin_amdgcn_raw_buffer_store_b16(v.b, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ uint32_t bld_u32(rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ uint8_t bld_u8(rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *__restrict__ base, uint32_t byte_off) {
+  return *(const T *)((const char *)base + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(T *__restrict__ base, uint32_t byte_off, T v) {
+  *(T *)((char *)base + byte_off) = v;
+}
+
+// One source plane gz: byte bases of the plane in the state buffer (field 0)
+// and in the normals buffer (n0); field q sits q field strides further.
+struct PlaneSrc {
+  const char *f;      // state plane, field 0
+  const char *n;      // normals plane, n0
+  uint32_t pbS, pb4;  // plane bytes (storage type, float)
+  long long fsS, fs4; // field stride in bytes (storage type, float)
+};
+
 template <int NC, typename S>
-__device__ __forceinline__ void load_src(const CFieldsT<S> &src, const float *__restrict__ n0a,
-                                         const float *__restrict__ n1a,
-                                         const float *__restrict__ n2a,
-                                         const float *__restrict__ ka, int id, bool use_n,
-                                         const Params &p, Src<NC> &s) {
+__device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint32_t off4,
+                                         bool use_n, const Params &p, Src<NC> &s) {
 #pragma unroll
-  for (int q = 0; q < 10; q++) s.m[q] = ldf(src.f[q], id, p, q);
+  for (int q = 0; q < (NC == 2 ? 11 : 10); q++) {
+    const float v = to_f(bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS));
+    if constexpr (sizeof(S) == 4) s.m[q] = v;
+    else s.m[q] = v * p.isc[q];
+  }
   if constexpr (NC == 2) {
-    s.m[10] = ldf(src.f[TSLB_PHI], id, p, TSLB_PHI);
     if (use_n) {
-      s.nn[0] = n0a[id];
-      s.nn[1] = n1a[id];
-      s.nn[2] = n2a[id];
-      s.kap = ka[id];
+      s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
+      s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
+      s.nn[2] = bld<float>(rsrc(ps.n + 2 * ps.fs4, ps.pb4), off4);
+      s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
     } else {
       s.nn[0] = s.nn[1] = s.nn[2] = s.kap = 0.f;
     }
@@ -479,13 +537,11 @@ __device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int 
 }
 
 template <int NC, bool MASK, bool ADD, bool PF, int WPE, typename S>
-__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(CFieldsT<S> src, FieldsT<S> dst, const float *__restrict__ n0a,
-                                                const float *__restrict__ n1a,
-                                                const float *__restrict__ n2a,
-                                                const float *__restrict__ ka,
-                                                const uint8_t *__restrict__ flags,
-                                                const uint32_t *__restrict__ links, Params p) {
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
+    const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
+    const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, Params p) {
   constexpr int NQ = NC == 2 ? 28 : 24;
+  constexpr int NF = NC == 2 ? 11 : 10;
   __shared__ float lds[NQ][NSLOT];
 
   const int tid = threadIdx.x;
@@ -500,7 +556,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   const int z1 = min(z0 + p.z_chunk, p.nz);
   const int x = x0 + tx, y = y0 + ty;
   const bool active = x < p.nx && y < p.ny;
-  const int plane = p.nx * p.ny;
+  const size_t plane = (size_t)p.nx * p.ny;
+  const size_t tiles_plane = (size_t)p.tiles_x * p.tiles_y;
+  const uint32_t pbS = (uint32_t)(plane * sizeof(S)), pb4 = (uint32_t)(plane * 4);
+  const long long fsS = p.fs * (long long)sizeof(S), fs4 = p.fs * 4;
 
   // own source column
   int ox = 0, oy = 0;
@@ -516,6 +575,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   int rx = 0, ry = 0;
   const bool rxy = tid < NRING && src_coord(x0 + rsx - 1, p.nx, p.px, rx) &&
                    src_coord(y0 + rsy - 1, p.ny, p.py, ry);
+  // in-plane element offsets (loop-invariant) and flag tiles of both columns
+  const uint32_t o_xy = (uint32_t)(ox + p.nx * oy), r_xy = (uint32_t)(rx + p.nx * ry);
+  const uint32_t d_xy = active ? (uint32_t)(x + p.nx * y) : 0u;
+  const int o_tile = (oy / TY) * p.tiles_x + ox / TX;
+  const int r_tile = (ry / TY) * p.tiles_x + rx / TX;
 
   // ghost-wall bounce-back bits of this column
   uint32_t wall_xy = 0;
@@ -534,45 +598,50 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       if (z == p.nz - 1) m |= dirmask(2, -1);
     }
     if constexpr (MASK) {
-      if (active && z >= 0 && z < p.nz) m |= links[x + p.nx * y + plane * z];
+      if (active && z >= 0 && z < p.nz) m |= bld_u32(rsrc(links + plane * z, pb4), d_xy * 4u);
     }
     return m;
   };
 
-  // source fetch of one plane: own column and (for tid < NRING) the ring slot
-  Src<NC> so, sr;
-  bool so_ok = false, sr_ok = false;
+  // source fetch of one plane: own column and (tid < NRING) the ring slot
   auto fetch = [&](int zs, Src<NC> &o, bool &ook, Src<NC> &r, bool &rok) {
     int gz = 0;
     const bool zok = src_coord(zs, p.nz, p.pz, gz);
-    const int fz = gz * p.tiles_x * p.tiles_y;
+    PlaneSrc ps;
+    ps.f = (const char *)src + (size_t)gz * pbS;
+    ps.n = (const char *)nrm + (size_t)gz * pb4;
+    ps.pbS = pbS;
+    ps.pb4 = pb4;
+    ps.fsS = fsS;
+    ps.fs4 = fs4;
+    const rsrc_t fl = rsrc(flags + tiles_plane * (size_t)gz, (uint32_t)tiles_plane);
+    const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
     ook = zok && oxy;
     if (ook) {
-      const int id = ox + p.nx * oy + plane * gz;
-      if constexpr (MASK) ook = (links[id] & kSolid) == 0u;
+      if constexpr (MASK) ook = (bld_u32(lk, o_xy * 4u) & kSolid) == 0u;
       bool un = false;
-      if constexpr (NC == 2) un = flags[fz + (oy / TY) * p.tiles_x + ox / TX] != 0;
-      if (ook) load_src<NC>(src, n0a, n1a, n2a, ka, id, un, p, o);
+      if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)o_tile) != 0;
+      if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), o_xy * 4u, un, p, o);
     }
     rok = zok && rxy;
     if (rok) {
-      const int id = rx + p.nx * ry + plane * gz;
-      if constexpr (MASK) rok = (links[id] & kSolid) == 0u;
+      if constexpr (MASK) rok = (bld_u32(lk, r_xy * 4u) & kSolid) == 0u;
       bool un = false;
-      if constexpr (NC == 2) un = flags[fz + (ry / TY) * p.tiles_x + rx / TX] != 0;
-      if (rok) load_src<NC>(src, n0a, n1a, n2a, ka, id, un, p, r);
+      if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)r_tile) != 0;
+      if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), r_xy * 4u, un, p, r);
     }
   };
+
+  // One source plane zs.  aA, aB and aC accumulate the destination planes
+  // zs-1 (completed and written here), zs and zs+1; bmA..bmC are their
+  // bounce-back masks.  The loop below renames the three accumulators instead
+  // of copying them.
+  // With PF the next plane's sources are loaded after the first barrier, so
+  // that they are in flight during the gather.
+  Src<NC> so, sr;
+  bool so_ok = false, sr_ok = false;
   if constexpr (PF) fetch(z0 - 1, so, so_ok, sr, sr_ok);
-
-  Acc aA, aB, aC;
-  acc_zero(aA);
-  acc_zero(aB);
-  uint32_t bmA = 0, bmB = plane_mask(z0 - 1);
-
-  for (int zs = z0 - 1; zs <= z1; zs++) {
-    int gz = 0;
-    src_coord(zs, p.nz, p.pz, gz);
+  auto body = [&](int zs, Acc &aA, Acc &aB, Acc &aC, uint32_t &bmA, uint32_t &bmB, uint32_t &bmC) {
     Src<NC> co, cr;
     bool co_ok, cr_ok;
     if constexpr (PF) {
@@ -580,7 +649,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       cr = sr;
       co_ok = so_ok;
       cr_ok = sr_ok;
-      if (zs + 1 <= z1) fetch(zs + 1, so, so_ok, sr, sr_ok);
     } else {
       fetch(zs, co, co_ok, cr, cr_ok);
     }
@@ -605,8 +673,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       eval_src<NC, ADD>(cr, p, sk, gq);
     }
     __syncthreads();
+    if constexpr (PF) {
+      if (zs + 1 <= z1) fetch(zs + 1, so, so_ok, sr, sr_ok);
+    }
 
-    const uint32_t bmC = plane_mask(zs + 1);
+    bmC = plane_mask(zs + 1);
     acc_zero(aC);
     if (active) {
       if ((bmA | bmB | bmC) == 0u) {
@@ -648,51 +719,72 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     // ---- finalize destination plane zs-1
     const int zd = zs - 1;
     if (active && zd >= z0 && zd < z1) {
-      const int id = x + p.nx * y + plane * zd;
+      const size_t zb = (size_t)zd * pbS;
+      const uint32_t offS = d_xy * (uint32_t)sizeof(S);
       if (MASK && (bmA & kSolid)) {
 #pragma unroll
-        for (int q = 0; q < TSLB_NFIELDS; q++)
-          if (q < 10 || NC == 2) dst.f[q][id] = src.f[q][id];
+        for (int q = 0; q < NF; q++)
+          bst(rsrc((char *)dst + zb + q * fsS, pbS), offS,
+              bld<S>(rsrc((const char *)src + zb + q * fsS, pbS), offS));
       } else {
         const float rho = 1.f + aA.s0;
-        const float inv = 1.f / rho;
+        // reciprocal with one Newton step (about 0.5 ulp)
+        float inv = __builtin_amdgcn_rcpf(rho);
+        inv = inv * fmaf(-rho, inv, 2.f);
         const float u = aA.sx * inv, v = aA.sy * inv, w = aA.sz * inv;
         const float t = aA.s0 * (1.f / 3.f);
-        stf(dst.f[0], id, aA.s0, p, 0);
-        stf(dst.f[1], id, u, p, 1);
-        stf(dst.f[2], id, v, p, 2);
-        stf(dst.f[3], id, w, p, 3);
-        stf(dst.f[4], id, aA.sxx - t - rho * u * u, p, 4);
-        stf(dst.f[5], id, aA.syy - t - rho * v * v, p, 5);
-        stf(dst.f[6], id, aA.szz - t - rho * w * w, p, 6);
-        stf(dst.f[7], id, aA.sxy - rho * u * v, p, 7);
-        stf(dst.f[8], id, aA.sxz - rho * u * w, p, 8);
-        stf(dst.f[9], id, aA.syz - rho * v * w, p, 9);
-        if constexpr (NC == 2) stf(dst.f[10], id, aA.phi, p, 10);
+        const float out[11] = {aA.s0,
+                               u,
+                               v,
+                               w,
+                               aA.sxx - t - rho * u * u,
+                               aA.syy - t - rho * v * v,
+                               aA.szz - t - rho * w * w,
+                               aA.sxy - rho * u * v,
+                               aA.sxz - rho * u * w,
+                               aA.syz - rho * v * w,
+                               aA.phi};
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+          const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
+          if constexpr (sizeof(S) == 4) bst(r, offS, from_f<S>(out[q]));
+          else bst(r, offS, from_f<S>(out[q] * p.sc[q]));
+        }
       }
     }
     __syncthreads();
-    aA = aB;
-    aB = aC;
-    bmA = bmB;
-    bmB = bmC;
+  };
+
+  Acc a0, a1, a2;
+  acc_zero(a0);
+  acc_zero(a1);
+  uint32_t m0 = 0, m1 = plane_mask(z0 - 1), m2 = 0;
+  for (int zs = z0 - 1; zs <= z1; zs += 3) {
+    body(zs, a0, a1, a2, m0, m1, m2);
+    if (zs + 1 > z1) break;
+    body(zs + 1, a1, a2, a0, m1, m2, m0);
+    if (zs + 2 > z1) break;
+    body(zs + 2, a2, a0, a1, m2, m0, m1);
   }
 }
 
-// K2 + curvature, LDS-tiled.  A 64x8 block marches a z-chunk.  It keeps five
-// phi planes of its (64+4)x(8+4) footprint and three normal planes of its
-// (64+2)x(8+2) footprint in LDS, writes n = grad(phi)/(|grad(phi)|+1e-9)
+// K2 + curvature, LDS-tiled.  A 64x8 block marches a z-chunk.  It keeps four
+// phi planes of its (64+4)x(8+4) footprint and four normal planes of its
+// (64+2)x(8+2) footprint in LDS rings, writes n = grad(phi)/(|grad(phi)|+1e-9)
 // (recursive_2c_TSLB3D.f90:555-571) and kappa = -div(n) at nodes past the
 // interface gate phi(1-phi)|n|^2 > 0.2 (0 elsewhere, :614-631).  Across a
 // non-periodic face phi is read at the nearest interior node and n is 0; n is
-// 0 on solid nodes.
+// 0 on solid nodes.  The phi plane three ahead is loaded into registers while
+// the current plane is computed.
 //
 // Sparse output: n and kappa are zero wherever phi is locally uniform (the
 // bulk of both phases), so each (tile, plane) gets a flag byte, 1 when any of
 // its n or kappa is nonzero; n and kappa are written only for flagged
 // tile-planes, and the fused pass reads them only there.  A plane whose phi
-// window z-1..z+1 is uniform skips the stencil.  `dense` writes everything.
+// window z-1..z+1 is uniform skips the stencil; uniformity of each loaded
+// plane is voted per wave from registers.  `dense` writes everything.
 constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
+constexpr int NWAVE = NTHR / 64;
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ phi,
                                                          float *__restrict__ n0,
@@ -702,8 +794,9 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
                                                          uint8_t *__restrict__ flags,
                                                          const uint32_t *__restrict__ links,
                                                          int has_mask, int dense, Params p) {
-  __shared__ float ph[5][NPSLOT];
-  __shared__ float nl[3][3][NSLOT];
+  __shared__ float ph[4][NPSLOT];
+  __shared__ float nl[4][3][NSLOT];
+  __shared__ float wv[NWAVE][2];  // per wave: plane uniform (1/0), its value
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -716,93 +809,145 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
   const int z1 = min(z0 + p.z_chunk, p.nz);
   const int x = x0 + tx, y = y0 + ty;
   const bool active = x < p.nx && y < p.ny;
-  const int plane = p.nx * p.ny;
-  const int tiles_plane = p.tiles_x * p.tiles_y;
+  const size_t plane = (size_t)p.nx * p.ny;
+  const size_t tiles_plane = (size_t)p.tiles_x * p.tiles_y;
   auto clampw = [](int g, int n, int per) -> int {
     if (g >= 0 && g < n) return g;
     return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
   };
-  // ring slots are indexed by (z - z0 + 6) mod 3 / mod 5 with z >= z0 - 2
-  auto mod3 = [z0](int z) { return (z - z0 + 6) % 3; };
-  auto mod5 = [z0](int z) { return (z - z0 + 10) % 5; };
-  auto load_phi = [&](int z) {
-    const int zz = clampw(z, p.nz, p.pz);
-    float *dstp = ph[mod5(z)];
-    for (int sl = tid; sl < NPSLOT; sl += NTHR) {
-      const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
-      const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
-      dstp[sl] = ldf(phi, gx + p.nx * gy + plane * zz, p, TSLB_PHI);
+  // ring slot of plane z >= z0 - 2
+  auto r4 = [z0](int z) { return (z - z0 + 4) & 3; };
+
+  // phi slots of this thread (tid, tid + NTHR) as in-plane byte offsets
+  const bool two = tid + NTHR < NPSLOT;
+  auto phi_off = [&](int sl) -> uint32_t {
+    const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
+    const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
+    return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
+  };
+  const uint32_t pofs0 = phi_off(tid), pofs1 = two ? phi_off(tid + NTHR) : 0u;
+  float pr0 = 0.f, pr1 = 0.f;
+  const uint32_t pbS = (uint32_t)(plane * sizeof(S)), pb4 = (uint32_t)(plane * 4);
+  auto issue = [&](int z) {
+    const rsrc_t base = rsrc(phi + plane * (size_t)clampw(z, p.nz, p.pz), pbS);
+    pr0 = to_f(bld<S>(base, pofs0));
+    if (two) pr1 = to_f(bld<S>(base, pofs1));
+    if constexpr (sizeof(S) != 4) {
+      pr0 *= p.isc[TSLB_PHI];
+      pr1 *= p.isc[TSLB_PHI];
     }
   };
-  // block vote (after a barrier): is the loaded phi plane z one value?
-  auto uniform_plane = [&](int z) -> bool {
-    const float *pl = ph[mod5(z)];
-    const float v0 = pl[0];
-    int diff = 0;
-    for (int sl = tid; sl < NPSLOT; sl += NTHR) diff |= (pl[sl] != v0);
-    return __syncthreads_or(diff) == 0;
+  // store the loaded plane z and post this wave's vote: all its values equal?
+  auto commit = [&](int z) {
+    float *d = ph[r4(z)];
+    d[tid] = pr0;
+    if (two) d[tid + NTHR] = pr1;
+    const float v = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pr0)));
+    const bool all = __all(pr0 == v && (!two || pr1 == v));
+    if ((tid & 63) == 0) {
+      wv[tid >> 6][0] = all ? 1.f : 0.f;
+      wv[tid >> 6][1] = v;
+    }
   };
+  // after a barrier: is the plane of the last commit one value (and which)?
+  auto read_vote = [&](float &val) -> bool {
+    const float v0 = wv[0][1];
+    bool u = true;
+#pragma unroll
+    for (int w = 0; w < NWAVE; w++) u = u && wv[w][0] != 0.f && wv[w][1] == v0;
+    val = v0;
+    return u;
+  };
+
+  // normal slots of this thread (tid, tid + NTHR < NSLOT)
+  const bool ntwo = tid + NTHR < NSLOT;
+  auto nslot = [&](int sl, int &c, uint32_t &xy, bool &ok) {
+    const int sx = sl % SW, sy = sl / SW;
+    int gx = 0, gy = 0;
+    ok = src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
+    c = (sy + 1) * PW + (sx + 1);
+    xy = ok ? (uint32_t)(gx + p.nx * gy) * 4u : 0u;
+  };
+  int nc0 = 0, nc1 = 0;
+  uint32_t nxy0 = 0, nxy1 = 0;
+  bool nok0 = false, nok1 = false;
+  nslot(tid, nc0, nxy0, nok0);
+  if (ntwo) nslot(tid + NTHR, nc1, nxy1, nok1);
+
   // normals of plane z on the (TX+2)x(TY+2) footprint, from phi planes z-1..z+1
   auto normals_plane = [&](int z, bool zero) {
-    int gz;
+    int gz = 0;
     const bool zin = src_coord(z, p.nz, p.pz, gz);
-    const int zs3 = mod3(z);
-    for (int sl = tid; sl < NSLOT; sl += NTHR) {
-      float a = 0.f, bb = 0.f, c = 0.f;
-      if (!zero) {
-        const int sx = sl % SW, sy = sl / SW;
-        int gx, gy;
-        bool ok = zin && src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
-        if (ok && has_mask) ok = (links[gx + p.nx * gy + plane * gz] & kSolid) == 0u;
+    const float *P0 = ph[r4(z - 1)], *P1 = ph[r4(z)], *P2 = ph[r4(z + 1)];
+    float(*dn)[NSLOT] = nl[r4(z)];
+    const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
+    auto one = [&](int sl, int c, uint32_t xy, bool okxy) {
+      float a = 0.f, bb = 0.f, cc = 0.f;
+      if (!zero && zin && okxy) {
+        bool ok = true;
+        if (has_mask) ok = (bld_u32(lk, xy) & kSolid) == 0u;
         if (ok) {
-          const float *pl[3] = {ph[mod5(z - 1)], ph[mod5(z)], ph[mod5(z + 1)]};
           auto get = [&](int dx, int dy, int dz) -> float {
-            return pl[dz + 1][(sy + 1 + dy) * PW + (sx + 1 + dx)];
+            const float *pl = dz < 0 ? P0 : (dz > 0 ? P2 : P1);
+            return pl[c + dy * PW + dx];
           };
           const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
-          const float den = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f;
-          a = gx_ / den;
-          bb = gy_ / den;
-          c = gz_ / den;
+          const float inv = 1.f / (sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f);
+          a = gx_ * inv;
+          bb = gy_ * inv;
+          cc = gz_ * inv;
         }
       }
-      nl[zs3][0][sl] = a;
-      nl[zs3][1][sl] = bb;
-      nl[zs3][2][sl] = c;
-    }
+      dn[0][sl] = a;
+      dn[1][sl] = bb;
+      dn[2][sl] = cc;
+    };
+    one(tid, nc0, nxy0, nok0);
+    if (ntwo) one(tid + NTHR, nc1, nxy1, nok1);
   };
+
   // uniformity (u*) and value (v*) of phi planes z-2 .. z+2 for the current z
   bool um2, um1, u0, up1, up2;
   float vm2, vm1, v0, vp1, vp2;
-  for (int z = z0 - 2; z <= z0 + 1; z++) load_phi(z);
-  __syncthreads();
-  um2 = uniform_plane(z0 - 2);
-  vm2 = ph[mod5(z0 - 2)][0];
-  um1 = uniform_plane(z0 - 1);
-  vm1 = ph[mod5(z0 - 1)][0];
-  u0 = uniform_plane(z0);
-  v0 = ph[mod5(z0)][0];
-  up1 = uniform_plane(z0 + 1);
-  vp1 = ph[mod5(z0 + 1)][0];
+  bool us[4];
+  float vs[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    issue(z0 - 2 + k);
+    commit(z0 - 2 + k);
+    __syncthreads();
+    us[k] = read_vote(vs[k]);
+    __syncthreads();
+  }
+  um2 = us[0];
+  vm2 = vs[0];
+  um1 = us[1];
+  vm1 = vs[1];
+  u0 = us[2];
+  v0 = vs[2];
+  up1 = us[3];
+  vp1 = vs[3];
   normals_plane(z0 - 1, um2 && um1 && u0 && vm2 == vm1 && vm1 == v0);
+  normals_plane(z0, um1 && u0 && up1 && vm1 == v0 && v0 == vp1);
+  issue(z0 + 2);
+  __syncthreads();
   for (int z = z0; z < z1; z++) {
-    load_phi(z + 2);
+    commit(z + 2);
+    if (z + 3 <= z1 + 1) issue(z + 3);  // planes up to z1 + 1 are read
     __syncthreads();
-    up2 = uniform_plane(z + 2);
-    vp2 = ph[mod5(z + 2)][0];
+    up2 = read_vote(vp2);
     normals_plane(z + 1, u0 && up1 && up2 && v0 == vp1 && vp1 == vp2);
-    const bool zero_z = um1 && u0 && up1 && vm1 == v0 && v0 == vp1;
-    if (z == z0) normals_plane(z0, zero_z);
     __syncthreads();
+    const bool zero_z = um1 && u0 && up1 && vm1 == v0 && v0 == vp1;
     int any = 0;
     float a = 0.f, bb = 0.f, c = 0.f, curv = 0.f;
     if (!zero_z && active) {
       const int own = (ty + 1) * SW + (tx + 1);
-      const int zm = mod3(z - 1), z0s = mod3(z), zp = mod3(z + 1);
+      const int zm = r4(z - 1), z0s = r4(z), zp = r4(z + 1);
       a = nl[z0s][0][own];
       bb = nl[z0s][1][own];
       c = nl[z0s][2][own];
-      const float f = ph[mod5(z)][(ty + 2) * PW + (tx + 2)];
+      const float f = ph[z0s][(ty + 2) * PW + (tx + 2)];
       const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
       if (ind > 0.2f) {
         auto nat = [&](int comp) {
@@ -821,13 +966,14 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
     // block-uniform: skip the vote for a uniform window
     const int flag = zero_z ? 0 : __syncthreads_or(any);
     if ((flag || dense) && active) {
-      const int id = x + p.nx * y + plane * z;
-      n0[id] = a;
-      n1[id] = bb;
-      n2[id] = c;
-      kap[id] = curv;
+      const size_t zb = plane * (size_t)z;
+      const uint32_t off = (uint32_t)(x + p.nx * y) * 4u;
+      bst(rsrc(n0 + zb, pb4), off, a);
+      bst(rsrc(n1 + zb, pb4), off, bb);
+      bst(rsrc(n2 + zb, pb4), off, c);
+      bst(rsrc(kap + zb, pb4), off, curv);
     }
-    if (tid == 0) flags[z * tiles_plane + tile_y * p.tiles_x + tile_x] = (uint8_t)(flag || dense);
+    if (tid == 0) flags[tiles_plane * z + tile_y * p.tiles_x + tile_x] = (uint8_t)(flag || dense);
     um2 = um1;
     vm2 = vm1;
     um1 = u0;
@@ -1127,7 +1273,7 @@ struct tslb_ctx {
   double *part = nullptr, *red = nullptr;
   float *w_in = nullptr, *phi_in = nullptr;
   int zbc = 0, zbc_phi = 0;
-  int variant = 3;  // bit 0: prefetch the next source plane; bit 1: cap VGPRs for 2 blocks/CU
+  int variant = 2;  // bit 0: prefetch the next plane; bit 1: cap VGPRs at 128 (two blocks per CU)
   size_t bytes = 0;
   int timing = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -1239,7 +1385,6 @@ int launch_normals(tslb_ctx *c, int dense = 0) {
 int launch_fused(tslb_ctx *c) {
   const Params &p = c->kp;
   const int nblk = p.tiles_x * p.tiles_y * p.nzc;
-  const float *n0 = c->nrm, *n1 = c->nrm + c->fs, *n2 = c->nrm + 2 * c->fs, *ka = c->nrm + 3 * c->fs;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     e0 = take_event(c);
@@ -1252,13 +1397,13 @@ int launch_fused(tslb_ctx *c) {
   const bool w4 = (c->variant & 2) != 0;
   with_storage(c, [&](auto tag) {
     using S = decltype(tag);
-    CFieldsT<S> s = cfields_of<S>(c->buf[c->cur], c->fs);
-    FieldsT<S> d = fields_of<S>(c->buf[1 - c->cur], c->fs);
-#define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                          \
-  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
-                     n0, n1, n2, ka, c->flags, c->links, p)
-    // FP32 storage keeps the four kernel variants (prefetch x register cap)
-    // selectable for sweeps; reduced storage uses prefetch with the cap
+    const S *s = (const S *)c->buf[c->cur];
+    S *d = (S *)c->buf[1 - c->cur];
+#define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                      \
+  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, \
+                     d, c->nrm, c->flags, c->links, p)
+    // FP32 storage keeps four variants selectable for sweeps: prefetch of the
+    // next plane (bit 0) and a 128-VGPR cap for two blocks per CU (bit 1)
 #define TSLB_LAUNCH_V(NC, MASK, ADD)                                      \
   do {                                                                    \
     if constexpr (sizeof(S) == 4) {                                       \
@@ -1267,7 +1412,8 @@ int launch_fused(tslb_ctx *c) {
       else if (w4) TSLB_LAUNCH(NC, MASK, ADD, false, 4);                  \
       else TSLB_LAUNCH(NC, MASK, ADD, false, 1);                          \
     } else {                                                              \
-      TSLB_LAUNCH(NC, MASK, ADD, true, 4);                                \
+      if (pf) TSLB_LAUNCH(NC, MASK, ADD, true, 4);                        \
+      else TSLB_LAUNCH(NC, MASK, ADD, false, 4);                          \
     }                                                                     \
   } while (0)
     if (p.ncomp == 2) {
@@ -1347,6 +1493,9 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     return fail(TSLB_E_ARG, "lattice extents must be >= 1");
   const long long n = (long long)p->nx * p->ny * p->nz;
   if (n > 0x7fffffffLL) return fail(TSLB_E_ARG, "nx*ny*nz must fit in int32");
+  // kernels address one z-plane of a field through a buffer descriptor with
+  // 32-bit byte offsets
+  if ((long long)p->nx * p->ny * 4 > 0x7fffffffLL) return fail(TSLB_E_ARG, "nx*ny*4 bytes must fit in int32");
   if (p->ncomp != 1 && p->ncomp != 2) return fail(TSLB_E_ARG, "ncomp must be 1 or 2");
   if (!(p->tau > 0.5f)) return fail(TSLB_E_ARG, "tau must be > 0.5");
   if (p->storage != TSLB_STORAGE_FP32 && p->storage != TSLB_STORAGE_FP16 &&
@@ -1364,6 +1513,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   c->n = (size_t)n;
   c->fs = (c->n + 63) & ~(size_t)63;
   Params &k = c->kp;
+  k.fs = (long long)c->fs;
   k.nx = p->nx;
   k.ny = p->ny;
   k.nz = p->nz;
