@@ -84,10 +84,19 @@ constexpr uint32_t kAllDirs = (1u << 27) - 2u;  // bits 1..26
 constexpr uint32_t kSolid = 1u << 31;
 
 // tile geometry of k_fused
-constexpr int TX = 64, TY = 8;
+// (build-time override: -DTSLB_TILE_X=.. -DTSLB_TILE_Y=..)
+#ifndef TSLB_TILE_X
+#define TSLB_TILE_X 64
+#endif
+#ifndef TSLB_TILE_Y
+#define TSLB_TILE_Y 8
+#endif
+constexpr int TX = TSLB_TILE_X, TY = TSLB_TILE_Y;
+static_assert(TX % 64 == 0 || (TX * TY) % 64 == 0, "blocks are whole wavefronts");
 constexpr int SW = TX + 2, SH = TY + 2, NSLOT = SW * SH;
 constexpr int NRING = 2 * SW + 2 * TY;
 constexpr int NTHR = TX * TY;
+static_assert(NRING <= NTHR && NSLOT <= 2 * NTHR, "one ring slot and at most two footprint slots per thread");
 
 struct Params {
   int nx, ny, nz;
@@ -99,6 +108,7 @@ struct Params {
   float fx, fy, fz, lin_a;
   int z_chunk, tiles_x, tiles_y, nzc;
   long long fs;             // element stride between the fields of a state buffer
+  int gwall;                // 1: walls absorb phase populations (jet boundary)
   float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
   float isc[TSLB_NFIELDS];  // 1 / sc
 };
@@ -199,6 +209,24 @@ struct PopSink {
     else if constexpr (L == 5) h5 = v;
     else if constexpr (L == 6) h6 = v;
     else lds[qslot(L)][slot] = v;
+  }
+  // phase populations 1..4 (in-plane movers) go to LDS slots 24..27
+  template <int G>
+  __device__ __forceinline__ void putg(float v) {
+    lds[23 + G][slot] = v;
+  }
+};
+// All 27 (+7) populations of one node into registers (boundary kernels).
+struct ArraySink {
+  float h[27];
+  float g[7];
+  template <int L>
+  __device__ __forceinline__ void put(float v) {
+    h[L] = v;
+  }
+  template <int G>
+  __device__ __forceinline__ void putg(float v) {
+    g[G] = v;
   }
 };
 
@@ -398,9 +426,13 @@ struct PlaneSrc {
   long long fsS, fs4; // field stride in bytes (storage type, float)
 };
 
+// Loads of one source node.  offS / off4 are in-plane byte offsets; off4 is
+// kOOB where the tile-plane has no interface (the buffer then returns 0
+// without a memory access).
+constexpr uint32_t kOOB = 0x80000000u;
 template <int NC, typename S>
 __device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint32_t off4,
-                                         bool use_n, const Params &p, Src<NC> &s) {
+                                         const Params &p, Src<NC> &s) {
 #pragma unroll
   for (int q = 0; q < (NC == 2 ? 11 : 10); q++) {
     const float v = to_f(bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS));
@@ -408,19 +440,15 @@ __device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint
     else s.m[q] = v * p.isc[q];
   }
   if constexpr (NC == 2) {
-    if (use_n) {
-      s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
-      s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
-      s.nn[2] = bld<float>(rsrc(ps.n + 2 * ps.fs4, ps.pb4), off4);
-      s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
-    } else {
-      s.nn[0] = s.nn[1] = s.nn[2] = s.kap = 0.f;
-    }
+    s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
+    s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
+    s.nn[2] = bld<float>(rsrc(ps.n + 2 * ps.fs4, ps.pb4), off4);
+    s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
   }
 }
 
-template <int NC, bool ADD>
-__device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, PopSink &h, float *gq) {
+template <int NC, bool ADD, typename Sink>
+__device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, Sink &h, float *gq) {
   const float *m = s.m;
   if constexpr (NC == 2) {
     const float phi = m[10];
@@ -438,10 +466,10 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, PopS
     const float sh = p1g * p.sharp_c * phi * (1.f - phi);
     gq[0] = 0.25f * phi;
     const float base = p1g * phi;
-    h.lds[24][h.slot] = base * (1.f + 3.f * m[1]) + sh * n0;
-    h.lds[25][h.slot] = base * (1.f - 3.f * m[1]) - sh * n0;
-    h.lds[26][h.slot] = base * (1.f + 3.f * m[2]) + sh * n1;
-    h.lds[27][h.slot] = base * (1.f - 3.f * m[2]) - sh * n1;
+    h.template putg<1>(base * (1.f + 3.f * m[1]) + sh * n0);
+    h.template putg<2>(base * (1.f - 3.f * m[1]) - sh * n0);
+    h.template putg<3>(base * (1.f + 3.f * m[2]) + sh * n1);
+    h.template putg<4>(base * (1.f - 3.f * m[2]) - sh * n1);
     gq[5] = base * (1.f + 3.f * m[3]) + sh * n2;
     gq[6] = base * (1.f - 3.f * m[3]) - sh * n2;
   } else {
@@ -536,7 +564,7 @@ __device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int 
   acc_layer<CZ>(a, p);
 }
 
-template <int NC, bool MASK, bool ADD, bool PF, int WPE, typename S>
+template <int NC, bool MASK, bool ADD, int WPE, typename S>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
     const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
     const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, Params p) {
@@ -619,16 +647,16 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     ook = zok && oxy;
     if (ook) {
       if constexpr (MASK) ook = (bld_u32(lk, o_xy * 4u) & kSolid) == 0u;
-      bool un = false;
+      bool un = NC == 1;
       if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)o_tile) != 0;
-      if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), o_xy * 4u, un, p, o);
+      if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), un ? o_xy * 4u : kOOB, p, o);
     }
     rok = zok && rxy;
     if (rok) {
       if constexpr (MASK) rok = (bld_u32(lk, r_xy * 4u) & kSolid) == 0u;
-      bool un = false;
+      bool un = NC == 1;
       if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)r_tile) != 0;
-      if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), r_xy * 4u, un, p, r);
+      if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), un ? r_xy * 4u : kOOB, p, r);
     }
   };
 
@@ -636,22 +664,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   // zs-1 (completed and written here), zs and zs+1; bmA..bmC are their
   // bounce-back masks.  The loop below renames the three accumulators instead
   // of copying them.
-  // With PF the next plane's sources are loaded after the first barrier, so
-  // that they are in flight during the gather.
-  Src<NC> so, sr;
-  bool so_ok = false, sr_ok = false;
-  if constexpr (PF) fetch(z0 - 1, so, so_ok, sr, sr_ok);
   auto body = [&](int zs, Acc &aA, Acc &aB, Acc &aC, uint32_t &bmA, uint32_t &bmB, uint32_t &bmC) {
     Src<NC> co, cr;
     bool co_ok, cr_ok;
-    if constexpr (PF) {
-      co = so;
-      cr = sr;
-      co_ok = so_ok;
-      cr_ok = sr_ok;
-    } else {
-      fetch(zs, co, co_ok, cr, cr_ok);
-    }
+    fetch(zs, co, co_ok, cr, cr_ok);
     float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
     // ---- own source slot
@@ -673,9 +689,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       eval_src<NC, ADD>(cr, p, sk, gq);
     }
     __syncthreads();
-    if constexpr (PF) {
-      if (zs + 1 <= z1) fetch(zs + 1, so, so_ok, sr, sr_ok);
-    }
 
     bmC = plane_mask(zs + 1);
     acc_zero(aC);
@@ -707,9 +720,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           if (!((bmB >> 2) & 1u)) s += lds[25][(ty + 1) * SW + tx + 2];
           if (!((bmB >> 3) & 1u)) s += lds[26][ty * SW + tx + 1];
           if (!((bmB >> 4) & 1u)) s += lds[27][(ty + 2) * SW + tx + 1];
+          if (!p.gwall) {
 #pragma unroll
-          for (int l = 1; l <= 6; l++)
-            if ((bmB >> l) & 1u) s += gown[kOPP[l]];
+            for (int l = 1; l <= 6; l++)
+              if ((bmB >> l) & 1u) s += gown[kOPP[l]];
+          }
           aB.phi += s;
           if (!((bmC >> 5) & 1u)) aC.phi += g5;
         }
@@ -784,6 +799,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 // window z-1..z+1 is uniform skips the stencil; uniformity of each loaded
 // plane is voted per wave from registers.  `dense` writes everything.
 constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
+static_assert(NPSLOT <= 2 * NTHR, "at most two phi footprint slots per thread");
 constexpr int NWAVE = NTHR / 64;
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ phi,
@@ -1042,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_prepush(const S *__restrict__ phi,
     int sid = ok ? sx + p.nx * sy + p.nx * p.ny * sz : 0;
     if (ok && has_mask && (links[sid] & kSolid)) ok = false;
     if (!ok) {
-      s += own[kOPP[l]];
+      if (!p.gwall) s += own[kOPP[l]];
     } else {
       const int a = (l - 1) / 2;
       const float sgn = (l & 1) ? 1.f : -1.f;
@@ -1099,6 +1115,123 @@ __global__ __launch_bounds__(256) void k_zplane_bc(FieldsT<S> f, const float *__
     else if (q == TSLB_W) stf(f.f[q], lo, w_in[id], p, q);
     else if (q == TSLB_PHI && phi_in) stf(f.f[q], lo, phi_in[id], p, q);
     else f.f[q][lo] = f.f[q][lo1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Turbulent-jet boundary (bcs_2c_turbulent_jet, boundary_cds_3D_module.f90:
+// 862-1049).  Planes 0 (inlet) and nz-1 (outlet) are walls; k_fused has
+// bounced back the populations that cross from them into planes 1 and nz-2
+// (and, with Params::gwall, let no phase population cross).  This pass
+// replaces, in moment space, each such population of a fluid node y by the
+// regularized boundary value
+//   h_l(y) = feq_l(rho(y), u(x)) + (1-omega) w_l a^neq_l(Pi(y), u(y)) - w_l,
+// x = y - c_l on the wall plane (:932-983, :991-1041), by subtracting its
+// bounce-back value h_opp(l)(y) from the shifted sums of y and adding the new
+// one, then refinishing the moments.  At the outlet u(x) takes w from the
+// plane below x (:992), and that w is stored on the outlet plane.  Phase
+// field: g_5(y) = phi(x)(1 + 3 w(x))/8 at the inlet (:985-988), g_6(y) =
+// phi(y)(1 - 3*0.05)/8 at the outlet (:1046-1048).  Populations whose x lies
+// across a non-periodic x/y face keep their bounce-back value.  One thread per
+// column and side; time-t moments from src, time-t+1 moments in dst.
+// ---------------------------------------------------------------------------
+template <int NC, bool ADD, typename S>
+__global__ __launch_bounds__(256) void k_jet_bc(const S *__restrict__ src, S *__restrict__ dst,
+                                                const float *__restrict__ nrm,
+                                                const uint8_t *__restrict__ flags,
+                                                const uint32_t *__restrict__ links, Params p) {
+  const int plane = p.nx * p.ny;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 2 * plane) return;
+  const int side = t / plane, xy = t - side * plane;
+  const int i = xy % p.nx, j = xy / p.nx;
+  const int zy = side ? p.nz - 2 : 1, zx = side ? p.nz - 1 : 0;
+  const int sz = side ? -1 : 1;  // c_z of the populations crossing into plane zy
+  auto at = [&](int q, int id) -> float { return ldf(src + (size_t)q * p.fs, id, p, q); };
+  if (side) {
+    // outlet wall plane: w(x) = w(x - e_z), boundary_cds_3D_module.f90:992
+    stf(dst + (size_t)TSLB_W * p.fs, xy + plane * zx, at(TSLB_W, xy + plane * zy), p, TSLB_W);
+  }
+  const int y = xy + plane * zy;
+  const uint32_t ly = links[y];
+  if (ly & kSolid) return;
+
+  // time-t state of y and its shifted post-collision populations, as k_fused
+  // evaluated them for the bounce-back
+  Src<NC> sy;
+#pragma unroll
+  for (int q = 0; q < (NC == 2 ? 11 : 10); q++) sy.m[q] = at(q, y);
+  if constexpr (NC == 2) {
+    const int tile = (j / TY) * p.tiles_x + i / TX;
+    const bool un = flags[(size_t)zy * p.tiles_x * p.tiles_y + tile] != 0;
+    sy.nn[0] = un ? nrm[y] : 0.f;
+    sy.nn[1] = un ? nrm[(size_t)p.fs + y] : 0.f;
+    sy.nn[2] = un ? nrm[2 * (size_t)p.fs + y] : 0.f;
+    sy.kap = un ? nrm[3 * (size_t)p.fs + y] : 0.f;
+  }
+  ArraySink own;
+  float gq[7];
+  eval_src<NC, ADD>(sy, p, own, gq);
+
+  // shifted sums of y from its finished moments
+  auto dat = [&](int q) -> float { return ldf(dst + (size_t)q * p.fs, y, p, q); };
+  const float s0 = dat(TSLB_RHO), rho = 1.f + s0;
+  const float u1 = dat(TSLB_U), v1 = dat(TSLB_V), w1 = dat(TSLB_W);
+  const float t3 = s0 * (1.f / 3.f);
+  float sm[10] = {s0,
+                 rho * u1,
+                 rho * v1,
+                 rho * w1,
+                 dat(TSLB_PXX) + t3 + rho * u1 * u1,
+                 dat(TSLB_PYY) + t3 + rho * v1 * v1,
+                 dat(TSLB_PZZ) + t3 + rho * w1 * w1,
+                 dat(TSLB_PXY) + rho * u1 * v1,
+                 dat(TSLB_PXZ) + rho * u1 * w1,
+                 dat(TSLB_PYZ) + rho * v1 * w1};
+  bool changed = false;
+#pragma unroll
+  for (int l = 1; l < 27; l++) {
+    if (kCZ[l] != sz || !((ly >> l) & 1u)) continue;
+    int xi = 0, xj = 0;
+    if (!src_coord(i - kCX[l], p.nx, p.px, xi) || !src_coord(j - kCY[l], p.ny, p.py, xj)) continue;
+    const int x = xi + p.nx * xj + plane * zx;
+    const float ux = at(TSLB_U, x), vx = at(TSLB_V, x);
+    const float wx = side ? at(TSLB_W, xi + p.nx * xj + plane * zy) : at(TSLB_W, x);
+    ArraySink bnd;
+    all_pops_mt<false>(sy.m, ux, vx, wx, p, bnd);
+    const float d = bnd.h[l] - own.h[kOPP[l]];
+    const float cx = (float)kCX[l], cy = (float)kCY[l], cz = (float)kCZ[l];
+    sm[0] += d;
+    sm[1] += cx * d;
+    sm[2] += cy * d;
+    sm[3] += cz * d;
+    sm[4] += cx * cx * d;
+    sm[5] += cy * cy * d;
+    sm[6] += cz * cz * d;
+    sm[7] += cx * cy * d;
+    sm[8] += cx * cz * d;
+    sm[9] += cy * cz * d;
+    changed = true;
+  }
+  if (changed) {
+    const float r = 1.f + sm[0];
+    const float inv = 1.f / r;
+    const float u = sm[1] * inv, v = sm[2] * inv, w = sm[3] * inv;
+    const float tt = sm[0] * (1.f / 3.f);
+    const float out[10] = {sm[0], u, v, w, sm[4] - tt - r * u * u, sm[5] - tt - r * v * v,
+                           sm[6] - tt - r * w * w, sm[7] - r * u * v, sm[8] - r * u * w, sm[9] - r * v * w};
+#pragma unroll
+    for (int q = 0; q < 10; q++) stf(dst + (size_t)q * p.fs, y, out[q], p, q);
+  }
+  if constexpr (NC == 2) {
+    float add;
+    if (!side) {
+      const int x = xy + plane * zx;
+      add = 0.125f * at(TSLB_PHI, x) * (1.f + 3.f * at(TSLB_W, x));
+    } else {
+      add = 0.125f * sy.m[TSLB_PHI] * (1.f - 3.f * 0.05f);
+    }
+    stf(dst + (size_t)TSLB_PHI * p.fs, y, dat(TSLB_PHI) + add, p, TSLB_PHI);
   }
 }
 
@@ -1273,7 +1406,9 @@ struct tslb_ctx {
   double *part = nullptr, *red = nullptr;
   float *w_in = nullptr, *phi_in = nullptr;
   int zbc = 0, zbc_phi = 0;
-  int variant = 2;  // bit 0: prefetch the next plane; bit 1: cap VGPRs at 128 (two blocks per CU)
+  int jet = 0;     // turbulent-jet inlet/outlet planes (k_jet_bc after k_fused)
+  int zwalls = 0;  // bit 0: plane 0 all solid; bit 1: plane nz-1 all solid
+  int variant = 2;  // bit 1: cap VGPRs at 128 (two blocks per CU)
   size_t bytes = 0;
   int timing = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -1393,27 +1528,23 @@ int launch_fused(tslb_ctx *c) {
   }
   const bool add = p.ncomp == 1 && !p.shift &&
                    (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
-  const bool pf = (c->variant & 1) != 0;
   const bool w4 = (c->variant & 2) != 0;
   with_storage(c, [&](auto tag) {
     using S = decltype(tag);
     const S *s = (const S *)c->buf[c->cur];
     S *d = (S *)c->buf[1 - c->cur];
-#define TSLB_LAUNCH(NC, MASK, ADD, PF, WPE)                                                      \
-  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, PF, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, \
-                     d, c->nrm, c->flags, c->links, p)
-    // FP32 storage keeps four variants selectable for sweeps: prefetch of the
-    // next plane (bit 0) and a 128-VGPR cap for two blocks per CU (bit 1)
+#define TSLB_LAUNCH(NC, MASK, ADD, WPE)                                                          \
+  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
+                     c->nrm, c->flags, c->links, p)
+    // FP32 storage keeps both register budgets selectable for sweeps (capped
+    // at 128 VGPRs for two blocks per CU, or uncapped); reduced storage is capped
 #define TSLB_LAUNCH_V(NC, MASK, ADD)                                      \
   do {                                                                    \
     if constexpr (sizeof(S) == 4) {                                       \
-      if (pf && w4) TSLB_LAUNCH(NC, MASK, ADD, true, 4);                  \
-      else if (pf) TSLB_LAUNCH(NC, MASK, ADD, true, 1);                   \
-      else if (w4) TSLB_LAUNCH(NC, MASK, ADD, false, 4);                  \
-      else TSLB_LAUNCH(NC, MASK, ADD, false, 1);                          \
+      if (w4) TSLB_LAUNCH(NC, MASK, ADD, 4);                              \
+      else TSLB_LAUNCH(NC, MASK, ADD, 1);                                 \
     } else {                                                              \
-      if (pf) TSLB_LAUNCH(NC, MASK, ADD, true, 4);                        \
-      else TSLB_LAUNCH(NC, MASK, ADD, false, 4);                          \
+      TSLB_LAUNCH(NC, MASK, ADD, 4);                                      \
     }                                                                     \
   } while (0)
     if (p.ncomp == 2) {
@@ -1438,6 +1569,24 @@ int launch_fused(tslb_ctx *c) {
       int rc = drain_timing(c);
       if (rc) return rc;
     }
+  }
+  if (c->jet) {
+    const int plane = p.nx * p.ny;
+    const bool add1 = p.ncomp == 1 && !p.shift && (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
+    with_storage(c, [&](auto tag) {
+      using S = decltype(tag);
+      const S *s = (const S *)c->buf[c->cur];
+      S *d = (S *)c->buf[1 - c->cur];
+      const dim3 g((unsigned)((2 * plane + 255) / 256));
+      if (p.ncomp == 2)
+        hipLaunchKernelGGL((k_jet_bc<2, false, S>), g, dim3(256), 0, c->stream, s, d, c->nrm, c->flags, c->links, p);
+      else if (add1)
+        hipLaunchKernelGGL((k_jet_bc<1, true, S>), g, dim3(256), 0, c->stream, s, d, c->nrm, c->flags, c->links, p);
+      else
+        hipLaunchKernelGGL((k_jet_bc<1, false, S>), g, dim3(256), 0, c->stream, s, d, c->nrm, c->flags, c->links, p);
+      return 0;
+    });
+    HIPCK(hipGetLastError());
   }
   if (p.ncomp == 1) {
     // phi is carried unchanged by one-component runs
@@ -1529,9 +1678,23 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.fy = p->force[1];
   k.fz = p->force[2];
   k.lin_a = p->lin_force;
-  k.z_chunk = p->nz < 32 ? p->nz : 32;
   k.tiles_x = (p->nx + TX - 1) / TX;
   k.tiles_y = (p->ny + TY - 1) / TY;
+  {
+    // z-chunks of 64 planes (measured best at 512^3 among 32, 64 and the
+    // whole depth), shorter when that would leave fewer than two blocks per CU
+    int ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, p->device) == hipSuccess && prop.multiProcessorCount > 0)
+      ncu = prop.multiProcessorCount;
+    const long long tiles = (long long)k.tiles_x * k.tiles_y;
+    int zc = 64;
+    while (zc > 8 && tiles * ((p->nz + zc - 1) / zc) < 2LL * ncu) zc /= 2;
+    if (const char *v = getenv("TSLB_ZCHUNK")) zc = atoi(v);
+    if (zc < 1) zc = 1;
+    if (zc > p->nz) zc = p->nz;
+    k.z_chunk = zc;
+  }
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
   // storage scales: FP16 keeps rho-1 (x64) and Pi^neq (x4096) in its normal
   // range; BF16 has FP32's exponent range and FP32 needs none
@@ -1609,7 +1772,23 @@ int tslb_set_isfluid(tslb_ctx *c, const int32_t *isf) {
   if (rc) return rc;
   if (!isf) {
     c->has_mask = 0;
+    c->zwalls = 0;
+    c->jet = 0;
+    c->kp.gwall = 0;
     return TSLB_OK;
+  }
+  {
+    const size_t plane = (size_t)c->prm.nx * c->prm.ny;
+    int w0 = 1, w1 = 1;
+    for (size_t q = 0; q < plane; q++) {
+      if (isf[q]) w0 = 0;
+      if (isf[plane * (c->prm.nz - 1) + q]) w1 = 0;
+    }
+    c->zwalls = w0 | (w1 << 1);
+    if (c->jet && c->zwalls != 3) {
+      c->jet = 0;
+      c->kp.gwall = 0;
+    }
   }
   if (!c->links) {
     if ((rc = dev_alloc(c, (void **)&c->links, c->n * sizeof(uint32_t)))) return rc;
@@ -1748,6 +1927,45 @@ int tslb_step(tslb_ctx *c, int nsteps) {
   return TSLB_OK;
 }
 
+int tslb_set_jet_bc(tslb_ctx *c, int enable) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!enable) {
+    c->jet = 0;
+    c->kp.gwall = 0;
+    return TSLB_OK;
+  }
+  if (c->prm.nz < 3 || c->prm.periodic[2])
+    return fail(TSLB_E_ARG, "jet boundary needs nz >= 3 and a non-periodic z axis");
+  if (!c->has_mask || c->zwalls != 3)
+    return fail(TSLB_E_STATE, "jet boundary needs every node of planes k=1 and k=nz solid (tslb_set_isfluid)");
+  if (c->zbc) return fail(TSLB_E_STATE, "jet boundary and tslb_set_zplane_bc are exclusive");
+  c->jet = 1;
+  c->kp.gwall = 1;
+  return TSLB_OK;
+}
+
+int tslb_upload_plane(tslb_ctx *c, int field, int k, const float *data) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (field < 0 || field >= TSLB_NFIELDS || !data) return fail(TSLB_E_ARG, "bad field or pointer");
+  if (k < 0 || k >= c->prm.nz) return fail(TSLB_E_ARG, "plane index out of range");
+  const size_t plane = (size_t)c->prm.nx * c->prm.ny;
+  float *tmp = (float *)c->buf[1 - c->cur];
+  HIPCK(hipMemcpyAsync(tmp, data, plane * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_to_storage<S>, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, c->stream, tmp,
+                       field_ptr<S>(c->buf[c->cur], c->fs, field) + plane * (size_t)k, (int)plane, field,
+                       c->kp);
+    return 0;
+  });
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
 int tslb_set_zplane_bc(tslb_ctx *c, const float *w_in, const float *phi_in) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -1756,6 +1974,7 @@ int tslb_set_zplane_bc(tslb_ctx *c, const float *w_in, const float *phi_in) {
     return TSLB_OK;
   }
   if (c->prm.nz < 3 || c->prm.periodic[2]) return fail(TSLB_E_ARG, "z-plane BC needs nz >= 3 and a non-periodic z axis");
+  if (c->jet) return fail(TSLB_E_STATE, "tslb_set_zplane_bc and the jet boundary are exclusive");
   const size_t plane = (size_t)c->prm.nx * c->prm.ny;
   if (!c->w_in && (rc = dev_alloc(c, (void **)&c->w_in, plane * sizeof(float)))) return rc;
   if (!c->phi_in && (rc = dev_alloc(c, (void **)&c->phi_in, plane * sizeof(float)))) return rc;

@@ -36,7 +36,7 @@ program tslb_driver
   integer(c_size_t) :: free_b, total_b
   integer :: i, j, k, step, iframe2D, iu, narg
   character(len=256) :: nml
-  logical :: has_solid, jet
+  logical :: has_solid, jet, jet_root
   real(8) :: ts1, ts2, glups, kk, pi
 
   pi = 4.d0 * atan(1.d0)
@@ -49,6 +49,9 @@ program tslb_driver
   endif
   jet = trim(config) == 'jet'
   if (jet) periodic(3) = 0
+  ! the root driver's jet: walls on all six faces, regularized inlet/outlet
+  jet_root = trim(config) == 'jet_root'
+  if (jet_root) periodic = 0
 
   ! ---- parameters and allocation (recursive_2c_TSLB3D.f90:20-59) ------------
   call tslb_default_params(prm)
@@ -118,11 +121,34 @@ program tslb_driver
   case ('jet')
     ! inflow disk of radius 10 at z = 1 (boundary_cds_3D_module.f90:1233-1292)
     call inflow_plane(0)
+  case ('jet_root')
+    ! walls on all faces and an inflow disk of radius 20 on k = 1
+    ! (recursive_2c_TSLB3D.f90:79-104); the seeded generator replaces
+    ! random_number
+    isfluid(1, :, :) = 0
+    isfluid(nx, :, :) = 0
+    isfluid(:, 1, :) = 0
+    isfluid(:, ny, :) = 0
+    isfluid(:, :, 1) = 0
+    isfluid(:, :, nz) = 0
+    has_solid = .true.
+    w_in = 0.0
+    do j = 1, ny
+      do i = 1, nx
+        if ((real(i) - nx / 2.0)**2 + (real(j) - ny / 2.0)**2 <= 20.0**2) then
+          w_in(i, j) = uwall + 0.02 * sqrt(-2.0 * log(max(tslb_rand_noseeded(i, j, 1, 1), 1.0e-30))) * &
+            cos(2.0 * real(pi) * tslb_rand_noseeded(i, j, 1, 2))
+          phi(i, j, 1) = 1.0
+        endif
+      enddo
+    enddo
+    w(:, :, 1) = w_in
   case default
     write(0, *) 'unknown config ', trim(config)
     error stop 2
   end select
   if (has_solid) call tslb_check(tslb_set_isfluid(ctx, isfluid), 'tslb_set_isfluid')
+  if (jet_root) call tslb_check(tslb_set_jet_bc(ctx, 1), 'tslb_set_jet_bc')
   call tslb_check(tslb_upload_field(ctx, TSLB_RHO, rho), 'upload rho')
   call tslb_check(tslb_upload_field(ctx, TSLB_U, u), 'upload u')
   call tslb_check(tslb_upload_field(ctx, TSLB_V, v), 'upload v')
@@ -160,6 +186,13 @@ program tslb_driver
       call tslb_check(tslb_set_zplane_bc(ctx, c_loc(w_in), c_loc(phi_in)), 'tslb_set_zplane_bc')
       call end_timing2('LB', 'bcs_inflow')
     endif
+    if (jet_root .and. mod(step, 10) == 0) then
+      ! inflow refresh inside radius 10 (boundary_cds_3D_module.f90:866-877)
+      call start_timing2('LB', 'bcs_inflow')
+      call inflow_disk(step)
+      call tslb_check(tslb_upload_plane(ctx, TSLB_W, 0, w_in), 'tslb_upload_plane')
+      call end_timing2('LB', 'bcs_inflow')
+    endif
   enddo
   call tslb_check(tslb_sync(ctx), 'tslb_sync')
   ts2 = wall_seconds()
@@ -181,6 +214,23 @@ program tslb_driver
   call tslb_check(tslb_destroy(ctx), 'tslb_destroy')
 
 contains
+
+  ! jet_root inflow: redraw w inside radius 10 of the k = 1 plane, the same r
+  ! twice as the reference (boundary_cds_3D_module.f90:868-874); w between
+  ! radius 10 and 20 keeps its initial value
+  subroutine inflow_disk(istep)
+    integer, intent(in) :: istep
+    integer :: ii, jj
+    real :: rr
+    do jj = 1, ny
+      do ii = 1, nx
+        if ((real(ii) - nx / 2.0)**2 + (real(jj) - ny / 2.0)**2 <= 10.0**2) then
+          rr = tslb_rand_noseeded(ii, jj, 1, istep)
+          w_in(ii, jj) = uwall + 0.004 * sqrt(-2.0 * log(max(rr, 1.0e-30))) * cos(2.0 * real(pi) * rr)
+        endif
+      enddo
+    enddo
+  end subroutine inflow_disk
 
   ! inflow plane: w = uwall + 0.004 sqrt(-2 ln r) cos(2 pi r) inside the disk,
   ! with the same r twice as the reference (boundary_cds_3D_module.f90:1253-1255)

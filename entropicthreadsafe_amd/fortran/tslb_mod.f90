@@ -80,6 +80,17 @@ module tslb_mod
       type(c_ptr), value :: w_in    ! c_loc of nx*ny floats, or c_null_ptr to disable
       type(c_ptr), value :: phi_in  ! c_loc of nx*ny floats, or c_null_ptr
     end function
+    integer(c_int) function tslb_set_jet_bc(ctx, enable) bind(C, name='tslb_set_jet_bc')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+    integer(c_int) function tslb_upload_plane(ctx, field, k, a) bind(C, name='tslb_upload_plane')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field, k
+      real(c_float), intent(in) :: a(*)
+    end function
     integer(c_int) function tslb_reduce(ctx, out) bind(C, name='tslb_reduce')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

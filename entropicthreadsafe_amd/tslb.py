@@ -64,6 +64,8 @@ SIGNATURES = {
     "tslb_normals": (ctypes.c_int, [_P]),
     "tslb_collide_stream": (ctypes.c_int, [_P]),
     "tslb_set_zplane_bc": (ctypes.c_int, [_P, _FP, _FP]),
+    "tslb_set_jet_bc": (ctypes.c_int, [_P, ctypes.c_int]),
+    "tslb_upload_plane": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _FP]),
     "tslb_seed_noise": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     "tslb_reduce": (ctypes.c_int, [_P, _DP]),
     "tslb_sync": (ctypes.c_int, [_P]),
@@ -234,6 +236,19 @@ class Lattice:
         ph = None if phi_in is None else np.ascontiguousarray(phi_in, dtype=np.float32).reshape(-1)
         _check(self.lib.tslb_set_zplane_bc(self._h, _fptr(w), None if ph is None else _fptr(ph)),
                "tslb_set_zplane_bc")
+
+    def set_jet_bc(self, enable=True):
+        """Turbulent-jet inlet (plane 0) / outlet (plane nz-1) walls; see
+        tslb_set_jet_bc.  Call after set_isfluid and before init_equilibrium."""
+        _check(self.lib.tslb_set_jet_bc(self._h, int(bool(enable))), "tslb_set_jet_bc")
+
+    def upload_plane(self, name, k, arr):
+        """One z-plane k (0-based) of a field, arr of shape (ny, nx)."""
+        q = FIELDS.index(name) if isinstance(name, str) else int(name)
+        a = np.ascontiguousarray(arr, dtype=np.float32).reshape(-1)
+        if a.size != self.shape[1] * self.shape[2]:
+            raise ValueError("plane has the wrong size")
+        _check(self.lib.tslb_upload_plane(self._h, q, int(k), _fptr(a)), "tslb_upload_plane")
 
     def seed_noise(self, name, amp, l):
         """field += amp*(2 r - 1), r = rand_noseeded(i, j, k, l), on the device."""

@@ -139,6 +139,27 @@ int tslb_collide_stream(tslb_ctx *ctx);
  * (NULL keeps the copy from z=2). */
 int tslb_set_zplane_bc(tslb_ctx *ctx, const float *w_in, const float *phi_in);
 
+/* Turbulent-jet boundary, the root driver's bcs_2c_turbulent_jet
+ * (boundary_cds_3D_module.f90:862-1049): planes k=1 and k=nz are walls
+ * (every node solid in tslb_set_isfluid), z not periodic.  Each step, after
+ * the collide-stream, the 9 populations crossing from plane 1 into plane 2
+ * become the regularized inlet values
+ *   feq_l(rho(y), u(x)) + (1-omega) w_l a^neq_l(Pi(y), u(y)),  x = y - c_l,
+ * with u(x) the velocity stored on the inlet plane (set it with
+ * tslb_upload_plane); the outlet (plane nz) first copies w from plane nz-1,
+ * then does the same for the populations crossing into plane nz-1.  Phase
+ * field: walls neither reflect nor send g (g_5 into plane 2 is
+ * phi(x)(1+3w(x))/8, g_6 into plane nz-1 is phi(y)(1-0.15)/8).  Enable it
+ * before tslb_init_equilibrium so that the initial phi push follows the same
+ * wall rule.  Exclusive with tslb_set_zplane_bc.  enable = 0 disables it. */
+int tslb_set_jet_bc(tslb_ctx *ctx, int enable);
+
+/* Host -> device one z-plane k (0-based, nx*ny floats, i fastest) of a
+ * moment field, e.g. the inflow velocity w and phi on the inlet plane.
+ * Replaces `!$acc update device(w(...,1))` of the inflow refresh
+ * (boundary_cds_3D_module.f90:866-877). */
+int tslb_upload_plane(tslb_ctx *ctx, int field, int k, const float *data);
+
 /* Seeded noise: field += amp * (2 r - 1) with r = tslb_rand_noseeded(i, j, k, l)
  * at 1-based node coordinates, computed on the device.  Replaces the seeded
  * initial perturbation of initial_conditions

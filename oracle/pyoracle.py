@@ -50,6 +50,8 @@ def lib():
             getattr(L, f"orc_reduce_{sfx}").argtypes = [P, ctypes.POINTER(ctypes.c_double)]
             getattr(L, f"orc_feq_{sfx}").argtypes = [ct, ct, ct, ct, cp]
             getattr(L, f"orc_aneq_{sfx}").argtypes = [cp, ct, ct, ct, cp]
+            getattr(L, f"orc_set_jet_{sfx}").argtypes = [P, ctypes.c_int]
+            getattr(L, f"orc_jet_values_{sfx}").argtypes = [ct, ct, cp, cp, cp, cp]
         L.orc_rand_noseeded.restype = ctypes.c_float
         L.orc_rand_noseeded.argtypes = [ctypes.c_int] * 4
         L.orc_gauss_noseeded.restype = ctypes.c_float
@@ -88,6 +90,18 @@ def aneq(P6, u, v, w, dtype=np.float64):
     P = np.ascontiguousarray(P6, dtype=dtype)
     out = np.zeros(27, dtype=dtype)
     getattr(lib(), f"orc_aneq_{sfx}")(_ptr(P, ct), u, v, w, _ptr(out, ct))
+    return out
+
+
+def jet_values(omega, rho_y, P_y, u_y, u_x, dtype=np.float64):
+    """27 regularized jet-boundary values feq(rho_y, u_x) + (1-omega) w a^neq(P_y, u_y)."""
+    sfx, ct = ("d", ctypes.c_double) if dtype == np.float64 else ("f", ctypes.c_float)
+    P = np.ascontiguousarray(P_y, dtype=dtype)
+    uy = np.ascontiguousarray(u_y, dtype=dtype)
+    ux = np.ascontiguousarray(u_x, dtype=dtype)
+    out = np.zeros(27, dtype=dtype)
+    getattr(lib(), f"orc_jet_values_{sfx}")(omega, rho_y, _ptr(P, ct), _ptr(uy, ct), _ptr(ux, ct),
+                                            _ptr(out, ct))
     return out
 
 
@@ -131,6 +145,10 @@ class Oracle:
         a = np.ascontiguousarray(isf, dtype=np.int32).reshape(-1)
         assert a.size == self.n
         self._fn("set_isfluid")(self._h, _ptr(a, ctypes.c_int32))
+
+    def set_jet(self, enable=True):
+        """Turbulent-jet inlet (k=0) / outlet (k=nz-1) wall planes; set before init."""
+        self._fn("set_jet")(self._h, int(bool(enable)))
 
     def init(self, rho, u, v, w, phi=None, prepush=True):
         arrs = [np.ascontiguousarray(x, dtype=self.dtype).reshape(-1) for x in (rho, u, v, w)]

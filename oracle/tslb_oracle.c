@@ -18,6 +18,7 @@
  *     1C additive forcing      recursive_TSLB3D.f90:571-707
  *     1C/2C velocity-shift     TSLB_2C-mpi/lb_kernels_module.f90:585-587,769-771
  *   - bounce-back on solids    boundary_cds_3D_module.f90:10-75
+ *   - jet inlet / outlet       boundary_cds_3D_module.f90:862-1049
  *   - periodic ghost wrap      TSLB_newBCS/boundary_cds_3D_module.f90:1294-1877
  *   - counter-based RNG        TSLB_newBCS/vars_single_3D_module.f90:78-165
  *   - initial state (f=feq, g pre-pushed)  recursive_2c_TSLB3D.f90:106-287
@@ -80,6 +81,7 @@ enum { F_RHO = 0, F_U, F_V, F_W, F_PXX, F_PYY, F_PZZ, F_PXY, F_PXZ, F_PYZ, F_PHI
 
 typedef struct {
   int nx, ny, nz, ncomp, per[3], force_mode;
+  int jet;            /* turbulent-jet inlet/outlet planes (orc_set_jet) */
   real omega, sigma, sharp_c, fx, fy, fz, lin_a;
   size_t n, ng;
   real *f, *g;        /* [27][ng], [7][ng] */
@@ -479,6 +481,85 @@ static void collide_node(const ctx_t *c, size_t p, const real *shift, real *fo, 
   }
 }
 
+/* ---------------------------------------------------------------------- */
+/* Turbulent-jet boundary, bcs_2c_turbulent_jet (boundary_cds_3D_module.f90: */
+/* 862-1049).  Planes k = 0 (inlet) and k = nz-1 (outlet) are walls.  After  */
+/* K3 the solids bounce f back (:878-930) but not g: a g population pushed    */
+/* into a wall is lost, and a wall sends none (its slots keep their zero      */
+/* allocation value).  Then the 9 populations crossing from the inlet plane   */
+/* into plane 1 are replaced by regularized inlet values                      */
+/*   f_l(y) = feq_l(rho(y), u(x)) + (1-omega) w_l a^neq_l(Pi(y), u(y)),       */
+/* x = y - c_l on the inlet plane (:932-990), and g_5(y) = phi(x)(1+3w(x))/8  */
+/* (:992-993).  On the outlet plane w(x) = w(x - e_z) (:996), then the 9      */
+/* populations crossing into plane nz-2 take the same form (:997-1044) and    */
+/* g_6(y) = phi(y)(1 - 3*0.05)/8 (:1048-1049).  Destinations across a         */
+/* non-periodic x/y face keep their bounce-back value.                        */
+/* ---------------------------------------------------------------------- */
+static real jet_value(int l, real omega, real rho_y, const real *P_y, real uy, real vy, real wy,
+                      real ux, real vx, real wx) {
+  return feq_l(l, rho_y, ux, vx, wx) + ((real)1 - omega) * aneq_l(l, P_y, uy, vy, wy) * wgt(l);
+}
+static real jet_pop(const ctx_t *c, int l, size_t py, real ux, real vx, real wx) {
+  real P[6];
+  for (int q = 0; q < 6; q++) P[q] = c->m[F_PXX + q][py];
+  return jet_value(l, c->omega, c->m[F_RHO][py], P, c->m[F_U][py], c->m[F_V][py], c->m[F_W][py], ux,
+                   vx, wx);
+}
+/* the 27 jet values for destination moments (rho_y, P_y, u_y) and a wall
+ * velocity u_x (exposed for the pins against the reference's polynomials) */
+void FN(orc_jet_values)(real omega, real rho_y, const real *P_y, const real *u_y, const real *u_x,
+                        real *out) {
+  for (int l = 0; l < 27; l++)
+    out[l] = jet_value(l, omega, rho_y, P_y, u_y[0], u_y[1], u_y[2], u_x[0], u_x[1], u_x[2]);
+}
+
+/* in-plane wrap of a destination column; 0 across a non-periodic face */
+static int wrap_xy(const ctx_t *c, int *i, int *j) {
+  if (*i < 0 || *i >= c->nx) {
+    if (!c->per[0]) return 0;
+    *i = wrap1(*i, c->nx);
+  }
+  if (*j < 0 || *j >= c->ny) {
+    if (!c->per[1]) return 0;
+    *j = wrap1(*j, c->ny);
+  }
+  return 1;
+}
+
+void FN(orc_set_jet)(void *h, int enable) { ((ctx_t *)h)->jet = enable ? 1 : 0; }
+
+/* jet values of one inlet (k = 0) or outlet (k = nz-1) node x, push form */
+static void jet_push_node(ctx_t *c, int i, int j, int outlet) {
+  const int kx = outlet ? c->nz - 1 : 0, ky = outlet ? c->nz - 2 : 1, sz = outlet ? -1 : 1;
+  const size_t px = idx(c, i, j, kx);
+  const real ux = c->m[F_U][px], vx = c->m[F_V][px], wx = c->m[F_W][px];
+  for (int l = 1; l < 27; l++) {
+    if (CZ[l] != sz) continue;
+    int yi = i + CX[l], yj = j + CY[l];
+    if (!wrap_xy(c, &yi, &yj)) continue;
+    c->f[(size_t)l * c->ng + gidx(c, yi, yj, ky)] = jet_pop(c, l, idx(c, yi, yj, ky), ux, vx, wx);
+  }
+  if (c->ncomp == 2) {
+    const real p1g = (real)0.125;
+    if (!outlet)
+      c->g[(size_t)5 * c->ng + gidx(c, i, j, 1)] = p1g * c->m[F_PHI][px] * ((real)1 + (real)3 * wx);
+    else
+      c->g[(size_t)6 * c->ng + gidx(c, i, j, ky)] =
+          p1g * c->m[F_PHI][idx(c, i, j, ky)] * ((real)1 - (real)3 * (real)0.05);
+  }
+}
+
+static void jet_bc_push(ctx_t *c) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) c->m[F_W][idx(c, i, j, nz - 1)] = c->m[F_W][idx(c, i, j, nz - 2)];
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      jet_push_node(c, i, j, 0);
+      jet_push_node(c, i, j, 1);
+    }
+}
+
 /* K3 push on fluid nodes, then boundary resolve */
 static void k3_push(ctx_t *c) {
   const int nx = c->nx, ny = c->ny, nz = c->nz;
@@ -514,8 +595,10 @@ static void resolve(ctx_t *c, real *a, int nq) {
           int ri = si, rj = sj, rk = sk;
           int inside = wrap_src(c, &si, &sj, &sk);
           if (!inside || !c->isf[idx(c, si, sj, sk)]) {
-            /* halfway bounce-back: the population y sent towards s returns */
-            a[(size_t)l * c->ng + gp] = a[(size_t)OPP[l] * c->ng + gidx(c, ri, rj, rk)];
+            /* halfway bounce-back: the population y sent towards s returns;
+             * the jet boundary reflects f only */
+            if (c->jet && nq == 7) a[(size_t)l * c->ng + gp] = 0;
+            else a[(size_t)l * c->ng + gp] = a[(size_t)OPP[l] * c->ng + gidx(c, ri, rj, rk)];
           } else if (si != ri || sj != rj || sk != rk) {
             /* periodic wrap: the source pushed into its ghost neighbour */
             a[(size_t)l * c->ng + gp] =
@@ -533,6 +616,7 @@ void FN(orc_step)(void *h, int nsteps) {
     k3_push(c);
     resolve(c, c->f, 27);
     if (c->ncomp == 2) resolve(c, c->g, 7);
+    if (c->jet) jet_bc_push(c);
     k1_moments(c);
   }
 }
@@ -631,7 +715,21 @@ void FN(orc_pull_step)(void *h, int nsteps) {
             int inside = wrap_src(c, &si, &sj, &sk);
             if (l > 0 && (!inside || !c->isf[idx(c, si, sj, sk)])) {
               fl[l] = c->pops[(size_t)OPP[l] * c->n + p];
-              if (l < 7 && c->ncomp == 2) gl[l] = c->gpops[(size_t)OPP[l] * c->n + p];
+              if (l < 7 && c->ncomp == 2)
+                gl[l] = c->jet ? (real)0 : c->gpops[(size_t)OPP[l] * c->n + p];
+              /* jet inlet / outlet: the source column x on the wall plane */
+              const int in_l = c->jet && k == 1 && CZ[l] == 1;
+              const int out_l = c->jet && k == nz - 2 && CZ[l] == -1;
+              int xi = i - CX[l], xj = j - CY[l];
+              if ((in_l || out_l) && wrap_xy(c, &xi, &xj)) {
+                const size_t px = idx(c, xi, xj, in_l ? 0 : nz - 1);
+                const real wx = in_l ? c->m[F_W][px] : c->m[F_W][idx(c, xi, xj, nz - 2)];
+                fl[l] = jet_pop(c, l, p, c->m[F_U][px], c->m[F_V][px], wx);
+                if (c->ncomp == 2 && l == 5)
+                  gl[l] = (real)0.125 * c->m[F_PHI][px] * ((real)1 + (real)3 * wx);
+                if (c->ncomp == 2 && l == 6)
+                  gl[l] = (real)0.125 * c->m[F_PHI][p] * ((real)1 - (real)3 * (real)0.05);
+              }
             } else {
               size_t sp = idx(c, si, sj, sk);
               fl[l] = c->pops[(size_t)l * c->n + sp];
@@ -652,6 +750,9 @@ void FN(orc_pull_step)(void *h, int nsteps) {
             nm[F_PHI][p] = c->m[F_PHI][p];
           }
         }
+    if (c->jet)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) nm[F_W][idx(c, i, j, nz - 1)] = c->m[F_W][idx(c, i, j, nz - 2)];
     for (int q = 0; q < NFIELD; q++) memcpy(c->m[q], nm[q], c->n * sizeof(real));
   }
   for (int q = 0; q < NFIELD; q++) free(nm[q]);
