@@ -138,7 +138,11 @@ def main():
     t_fused = fused_ms / max(fused_n, 1) / 1e3
     achieved = bytes_per_launch / t_fused / 1e9
     t_step = (fused_ms + norm_ms) / max(fused_n, 1) / 1e3
-    traffic = load_traffic(os.path.join(ROOT, "profiles", "fused_2c_512_traffic.json"))
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
+    # with the copy kernel), for the default size and FP32 storage only
+    traffic = None
+    if n == 512 and args.storage == "fp32":
+        traffic = load_traffic(os.path.join(ROOT, "profiles", f"fused_{args.workload}_traffic.json"))
     out = {
         "metric": METRIC,
         "value": round(mlups, 2),
