@@ -1862,6 +1862,88 @@ int tslb_download_moments(tslb_ctx *c, float *m) {
   return TSLB_OK;
 }
 
+// Moment-space checkpoint: header + the stored fields verbatim.
+struct CkptHeader {
+  char magic[8];  // "TSLBCKPT"
+  int32_t version, nx, ny, nz, ncomp, storage, elem_bytes, nfields;
+  int64_t step;
+  int32_t pad[4];
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
+
+int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!path) return fail(TSLB_E_ARG, "null path");
+  CkptHeader h;
+  memset(&h, 0, sizeof(h));
+  memcpy(h.magic, "TSLBCKPT", 8);
+  h.version = 1;
+  h.nx = c->prm.nx;
+  h.ny = c->prm.ny;
+  h.nz = c->prm.nz;
+  h.ncomp = c->prm.ncomp;
+  h.storage = c->prm.storage;
+  h.elem_bytes = (int32_t)c->es;
+  h.nfields = TSLB_NFIELDS;
+  h.step = step;
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return fail(TSLB_E_ARG, std::string("cannot open ") + path + " for writing");
+  const size_t bytes = c->n * c->es;
+  std::vector<char> host(bytes);
+  bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
+  for (int q = 0; ok && q < TSLB_NFIELDS; q++) {
+    hipError_t e = hipMemcpyAsync(host.data(), (char *)c->buf[c->cur] + (size_t)q * c->fs * c->es, bytes,
+                                  hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      fclose(fp);
+      return fail(TSLB_E_HIP, std::string("checkpoint copy: ") + hipGetErrorString(e));
+    }
+    ok = fwrite(host.data(), 1, bytes, fp) == bytes;
+  }
+  if (fclose(fp) != 0) ok = false;
+  if (!ok) return fail(TSLB_E_ARG, std::string("short write to ") + path);
+  return TSLB_OK;
+}
+
+int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!path) return fail(TSLB_E_ARG, "null path");
+  FILE *fp = fopen(path, "rb");
+  if (!fp) return fail(TSLB_E_ARG, std::string("cannot open ") + path);
+  CkptHeader h;
+  if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0 || h.version != 1) {
+    fclose(fp);
+    return fail(TSLB_E_ARG, std::string(path) + " is not a TSLB checkpoint");
+  }
+  if (h.nx != c->prm.nx || h.ny != c->prm.ny || h.nz != c->prm.nz || h.ncomp != c->prm.ncomp ||
+      h.storage != c->prm.storage || h.elem_bytes != (int32_t)c->es || h.nfields != TSLB_NFIELDS) {
+    fclose(fp);
+    return fail(TSLB_E_ARG, "checkpoint lattice, components or storage differ from the context");
+  }
+  const size_t bytes = c->n * c->es;
+  std::vector<char> host(bytes);
+  for (int q = 0; q < TSLB_NFIELDS; q++) {
+    if (fread(host.data(), 1, bytes, fp) != bytes) {
+      fclose(fp);
+      return fail(TSLB_E_ARG, std::string("truncated checkpoint ") + path);
+    }
+    hipError_t e = hipMemcpyAsync((char *)c->buf[c->cur] + (size_t)q * c->fs * c->es, host.data(), bytes,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      fclose(fp);
+      return fail(TSLB_E_HIP, std::string("checkpoint copy: ") + hipGetErrorString(e));
+    }
+  }
+  fclose(fp);
+  if (step) *step = h.step;
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
 int tslb_download_normals(tslb_ctx *c, float *n3) {
   int rc = check_ctx(c);
   if (rc) return rc;

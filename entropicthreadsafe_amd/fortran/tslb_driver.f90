@@ -24,8 +24,12 @@ program tslb_driver
   character(len=16) :: config = 'droplet'
   real :: radius = 10.0, u0 = 0.05, uwall = 0.05, noise = 1.0e-3
   character(len=256) :: dir_out = 'output/'
+  ! checkpoint (dump/out_m_.ckpt) every dumpstep steps; restart from a file
+  integer :: dumpstep = 0, stamp_vtk = 0
+  character(len=256) :: restart = ''
   namelist /tslb_run/ nx, ny, nz, ncomp, nsteps, stamp2D, periodic, force_mode, tau, sigma, &
-    sharp_c, fx, fy, fz, lin_force, config, radius, u0, uwall, noise, dir_out
+    sharp_c, fx, fy, fz, lin_force, config, radius, u0, uwall, noise, dir_out, dumpstep, &
+    stamp_vtk, restart
 
   type(tslb_params) :: prm
   type(c_ptr) :: ctx
@@ -34,7 +38,8 @@ program tslb_driver
   integer(c_int32_t), allocatable :: isfluid(:,:,:)
   real(c_double) :: red0(7), red(7), rss, ram
   integer(c_size_t) :: free_b, total_b
-  integer :: i, j, k, step, iframe2D, iu, narg
+  integer :: i, j, k, step, iframe2D, iframe3D, iu, narg, step0
+  integer(c_int64_t) :: ck_step
   character(len=256) :: nml
   logical :: has_solid, jet, jet_root
   real(8) :: ts1, ts2, glups, kk, pi
@@ -161,6 +166,13 @@ program tslb_driver
     call tslb_check(tslb_seed_noise(ctx, TSLB_W, noise, 33), 'noise w')
   endif
   call tslb_check(tslb_init_equilibrium(ctx, 1), 'tslb_init_equilibrium')
+  step0 = 0
+  if (len_trim(restart) > 0) then
+    ! moment-space restart in place of read_distros_1c_3d (print_module.f90:713-723)
+    call tslb_check(tslb_checkpoint_read(ctx, trim(restart) // c_null_char, ck_step), 'tslb_checkpoint_read')
+    step0 = int(ck_step)
+    write(6, '(a,a,a,i0)') 'restart from ', trim(restart), ' at step ', step0
+  endif
   if (jet) call tslb_check(tslb_set_zplane_bc(ctx, c_loc(w_in), c_loc(phi_in)), 'tslb_set_zplane_bc')
 
   call execute_command_line('mkdir -p ' // trim(dir_out))
@@ -169,13 +181,33 @@ program tslb_driver
 
   ! ---- time loop (recursive_2c_TSLB3D.f90:344-840) ---------------------------
   iframe2D = 0
+  iframe3D = 0
+  if (dumpstep > 0) call execute_command_line('mkdir -p dump')
   ts1 = wall_seconds()
-  do step = 1, nsteps
+  do step = step0 + 1, step0 + nsteps
     if (mod(step, stamp2D) == 0) then
       call start_timing2('IO', 'print2d')
       iframe2D = iframe2D + 1
       call print_raw_slices(iframe2D)
       call end_timing2('IO', 'print2d')
+    endif
+    if (stamp_vtk > 0) then
+      if (mod(step, stamp_vtk) == 0) then
+        call start_timing2('IO', 'print_vtk')
+        iframe3D = iframe3D + 1
+        call print_vti_phi(iframe3D)
+        call end_timing2('IO', 'print_vtk')
+      endif
+    endif
+    if (dumpstep > 0) then
+      if (mod(step, dumpstep) == 0) then
+        ! checkpoint in place of dump_distros_1c_3d (print_module.f90:700-711); it
+        ! holds the state before this step's update
+        call start_timing2('IO', 'dump')
+        call tslb_check(tslb_checkpoint_write(ctx, 'dump/out_m_.ckpt' // c_null_char, &
+          int(step - 1, c_int64_t)), 'tslb_checkpoint_write')
+        call end_timing2('IO', 'dump')
+      endif
     endif
     call start_timing2('LB', 'step')
     call tslb_check(tslb_step(ctx, 1), 'tslb_step')
@@ -214,6 +246,33 @@ program tslb_driver
   call tslb_check(tslb_destroy(ctx), 'tslb_destroy')
 
 contains
+
+  ! phi as VTK ImageData with appended raw Float32 data, in place of
+  ! print_vtk_phionly_sync (print_module.f90:919-930): out_phi_NNNNNN.vti
+  subroutine print_vti_phi(iframe)
+    integer, intent(in) :: iframe
+    character(len=512) :: fname
+    character(len=1024) :: head
+    character(len=128) :: ext
+    character(len=1), parameter :: nl = char(10)
+    integer :: u
+    integer(4) :: nbytes
+    call tslb_check(tslb_download_field(ctx, TSLB_PHI, phi), 'download phi')
+    write(fname, '(a,a,i6.6,a)') trim(dir_out), 'out_phi_', iframe, '.vti'
+    write(ext, '(i0,a,i0,a,i0,a,i0,a,i0,a,i0)') 1, ' ', nx, ' ', 1, ' ', ny, ' ', 1, ' ', nz
+    head = '<?xml version="1.0"?>' // nl // &
+      '<VTKFile type="ImageData" version="0.1" byte_order="LittleEndian" header_type="UInt32">' // nl // &
+      '  <ImageData WholeExtent="' // trim(ext) // '" Origin="0 0 0" Spacing="1 1 1">' // nl // &
+      '    <Piece Extent="' // trim(ext) // '">' // nl // &
+      '      <PointData Scalars="phi">' // nl // &
+      '        <DataArray type="Float32" Name="phi" NumberOfComponents="1" format="appended" offset="0"/>' // nl // &
+      '      </PointData>' // nl // '    </Piece>' // nl // '  </ImageData>' // nl // &
+      '  <AppendedData encoding="raw">' // nl // '   _'
+    nbytes = int(4, 4) * int(nx, 4) * int(ny, 4) * int(nz, 4)
+    open(newunit=u, file=trim(fname), status='replace', action='write', access='stream', form='unformatted')
+    write(u) trim(head), nbytes, phi, nl // '  </AppendedData>' // nl // '</VTKFile>' // nl
+    close(u)
+  end subroutine print_vti_phi
 
   ! jet_root inflow: redraw w inside radius 10 of the k = 1 plane, the same r
   ! twice as the reference (boundary_cds_3D_module.f90:868-874); w between

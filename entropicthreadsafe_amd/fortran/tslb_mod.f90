@@ -91,6 +91,18 @@ module tslb_mod
       integer(c_int), value :: field, k
       real(c_float), intent(in) :: a(*)
     end function
+    integer(c_int) function tslb_checkpoint_write(ctx, path, step) bind(C, name='tslb_checkpoint_write')
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: step
+    end function
+    integer(c_int) function tslb_checkpoint_read(ctx, path, step) bind(C, name='tslb_checkpoint_read')
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), intent(out) :: step
+    end function
     integer(c_int) function tslb_reduce(ctx, out) bind(C, name='tslb_reduce')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

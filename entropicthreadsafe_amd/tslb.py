@@ -65,6 +65,8 @@ SIGNATURES = {
     "tslb_collide_stream": (ctypes.c_int, [_P]),
     "tslb_set_zplane_bc": (ctypes.c_int, [_P, _FP, _FP]),
     "tslb_set_jet_bc": (ctypes.c_int, [_P, ctypes.c_int]),
+    "tslb_checkpoint_write": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int64]),
+    "tslb_checkpoint_read": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "tslb_upload_plane": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _FP]),
     "tslb_seed_noise": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     "tslb_reduce": (ctypes.c_int, [_P, _DP]),
@@ -236,6 +238,18 @@ class Lattice:
         ph = None if phi_in is None else np.ascontiguousarray(phi_in, dtype=np.float32).reshape(-1)
         _check(self.lib.tslb_set_zplane_bc(self._h, _fptr(w), None if ph is None else _fptr(ph)),
                "tslb_set_zplane_bc")
+
+    def checkpoint_write(self, path, step=0):
+        """Stored state verbatim (exact restart); see tslb_checkpoint_write."""
+        _check(self.lib.tslb_checkpoint_write(self._h, os.fsencode(path), int(step)),
+               "tslb_checkpoint_write")
+
+    def checkpoint_read(self, path):
+        """Restore a checkpoint written for the same lattice; returns its step."""
+        st = ctypes.c_int64(0)
+        _check(self.lib.tslb_checkpoint_read(self._h, os.fsencode(path), ctypes.byref(st)),
+               "tslb_checkpoint_read")
+        return int(st.value)
 
     def set_jet_bc(self, enable=True):
         """Turbulent-jet inlet (plane 0) / outlet (plane nz-1) walls; see

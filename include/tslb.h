@@ -104,6 +104,17 @@ int tslb_upload_field(tslb_ctx *ctx, int field, const float *data);
 int tslb_download_moments(tslb_ctx *ctx, float *m);
 int tslb_download_field(tslb_ctx *ctx, int field, float *data);
 
+/* Moment-space checkpoint, in place of the population dump and reload
+ * dump_distros_1c_3d / read_distros_1c_3d (print_module.f90:700-723).  The
+ * file holds a 64-byte header (magic "TSLBCKPT", version 1, nx, ny, nz,
+ * ncomp, storage, bytes per value, field count, the caller's step) and the
+ * 11 stored fields verbatim (storage type, field 0 as rho-1), so a run that
+ * reads it continues bit for bit as one that never stopped.  The isfluid map
+ * and plane BCs are not part of it: the caller sets them again.  Reading
+ * checks lattice, component count and storage against the context. */
+int tslb_checkpoint_write(tslb_ctx *ctx, const char *path, int64_t step);
+int tslb_checkpoint_read(tslb_ctx *ctx, const char *path, int64_t *step);
+
 /* Colour-gradient normals n = grad(phi)/(|grad(phi)|+1e-9) of the current
  * state, 3*nx*ny*nz floats (nx, ny, nz components).  Valid after a step or
  * after tslb_normals().  Replaces `!$acc update host(normx,...)`. */

@@ -42,3 +42,48 @@ def test_driver_droplet_matches_python_binding(tmp_path):
         m = lat.download()
     assert np.array_equal(xy, m[10][n // 2 - 1])
     assert np.array_equal(uxz, m[1][:, n // 2 - 1, :])
+
+
+def read_vti_float32(path):
+    """Appended raw Float32 data of a VTK ImageData file (UInt32 size header)."""
+    raw = path.read_bytes()
+    start = raw.index(b"_", raw.index(b"<AppendedData")) + 1
+    nbytes = int(np.frombuffer(raw[start:start + 4], dtype="<u4")[0])
+    return np.frombuffer(raw[start + 4:start + 4 + nbytes], dtype="<f4")
+
+
+@pytest.mark.gpu
+def test_driver_restart_is_exact_and_vti_matches(tmp_path):
+    """dumpstep writes the moment-space checkpoint; a run restarted from it
+    reaches the same state bit for bit as the uninterrupted run.  The phi VTI
+    of a stamp step holds the full field."""
+    n = 24
+    base = (" nx=24, ny=24, nz=24, ncomp=2, stamp2D=100, tau=0.8, sigma=0.02, sharp_c=2.1,\n"
+            " config='droplet', radius=6.0, noise=1.0e-3")
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    # the checkpoint of step 10 holds the state after 9 steps; both runs write
+    # the phi VTI of step 15, the state after 14 steps
+    (a / "in.nml").write_text(f"&tslb_run\n{base}, nsteps=15, dumpstep=10, stamp_vtk=15\n/\n")
+    res = subprocess.run([DRIVER, "in.nml"], cwd=a, capture_output=True, text=True, timeout=120)
+    print(res.stdout[-1500:], res.stderr[-1500:])
+    assert res.returncode == 0
+    ck = a / "dump" / "out_m_.ckpt"
+    assert ck.exists()
+    (b / "in.nml").write_text(f"&tslb_run\n{base}, nsteps=6, stamp_vtk=15, restart='{ck}'\n/\n")
+    res = subprocess.run([DRIVER, "in.nml"], cwd=b, capture_output=True, text=True, timeout=120)
+    print(res.stdout[-1500:], res.stderr[-1500:])
+    assert res.returncode == 0
+    assert "restart from" in res.stdout and "at step 9" in res.stdout
+    pa = read_vti_float32(a / "output" / "out_phi_000001.vti")
+    pb = read_vti_float32(b / "output" / "out_phi_000001.vti")
+    assert pa.size == n ** 3
+    assert np.array_equal(pa, pb)
+    case = configs.droplet(n=n, radius=6.0, sigma=0.02, beta=0.7, noise=1e-3)
+    case.lattice["tau"] = 0.8
+    with Lattice(n, n, n, **case.lattice) as lat:
+        case.apply(lat)
+        lat.step(14)
+        phi = lat.download_field("phi")
+    assert np.array_equal(pa.reshape(n, n, n), phi)

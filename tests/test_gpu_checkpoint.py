@@ -1,0 +1,44 @@
+"""Moment-space checkpoint (tslb_checkpoint_write / _read): the stored state
+is saved verbatim, so a restarted run continues bit for bit."""
+import numpy as np
+import pytest
+
+from entropicthreadsafe_amd import configs
+from entropicthreadsafe_amd.tslb import Lattice, TslbError
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_restart_is_bitwise(tmp_path, storage):
+    case = configs.droplet(n=24, radius=6.0, sigma=0.02, beta=0.7, noise=1e-3)
+    ck = str(tmp_path / "state.ckpt")
+    with Lattice(24, 24, 24, storage=storage, **case.lattice) as lat:
+        case.apply(lat)
+        lat.step(7)
+        lat.checkpoint_write(ck, step=7)
+        lat.step(9)
+        straight = lat.download()
+    with Lattice(24, 24, 24, storage=storage, **case.lattice) as lat:
+        assert lat.checkpoint_read(ck) == 7
+        lat.step(9)
+        restarted = lat.download()
+    assert np.array_equal(straight, restarted)
+
+
+def test_checkpoint_rejects_another_lattice(tmp_path):
+    case = configs.tgv(n=16)
+    ck = str(tmp_path / "tgv.ckpt")
+    with Lattice(16, 16, 16, **case.lattice) as lat:
+        case.apply(lat)
+        lat.checkpoint_write(ck)
+    with Lattice(16, 16, 32, **case.lattice) as lat:
+        with pytest.raises(TslbError):
+            lat.checkpoint_read(ck)
+    with Lattice(16, 16, 16, storage="fp16", **case.lattice) as lat:
+        with pytest.raises(TslbError):
+            lat.checkpoint_read(ck)
+    (tmp_path / "junk.ckpt").write_bytes(b"not a checkpoint" * 8)
+    with Lattice(16, 16, 16, **case.lattice) as lat:
+        with pytest.raises(TslbError):
+            lat.checkpoint_read(str(tmp_path / "junk.ckpt"))
