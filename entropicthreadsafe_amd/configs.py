@@ -128,6 +128,22 @@ def droplet(n=256, radius=40.0, sigma=0.02, beta=0.7, noise=1e-3, tau=1.0):
                 noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)])
 
 
+def colour_droplet(n=32, radius=8.0, sigma=0.02, beta=0.7, tau=1.0, tau_b=1.0, noise=1e-3, width=1.0):
+    """Static droplet in the colour-gradient model (ncomp 3): colour field
+    psi = tanh((R - r)/width) (+1 inside, component A), seeded velocity noise
+    with l = 11, 22, 33; viscosities from tau (A) and tau_b (B)."""
+    i, j, k = _grid(n, n, n)
+    c = n / 2.0
+    r = np.sqrt((i + 1 - c) ** 2 + (j + 1 - c) ** 2 + (k + 1 - c) ** 2)
+    psi = np.tanh((radius - r) / width)
+    z = np.zeros_like(psi)
+    return Case("colour_droplet", (n, n, n),
+                dict(ncomp=3, tau=tau, tau_b=tau_b, beta=beta, sigma=sigma, periodic=(1, 1, 1)),
+                dict(rho=np.ones_like(psi), u=z, v=z.copy(), w=z.copy(), phi=psi),
+                noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)], prepush_phi=False,
+                notes=f"colour droplet R={radius}")
+
+
 def channel(nx=512, ny=256, nz=256, re_tau=180.0, u_tau=0.005, pert=0.05, ub_over_utau=15.6):
     """Pressure-driven channel, one component: x, z periodic, halfway
     bounce-back walls just outside y = 1 and y = ny (half height h = ny/2).

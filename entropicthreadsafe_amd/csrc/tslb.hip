@@ -109,6 +109,8 @@ struct Params {
   int z_chunk, tiles_x, tiles_y, nzc;
   long long fs;             // element stride between the fields of a state buffer
   int gwall;                // 1: walls absorb phase populations (jet boundary)
+  float nua, nub;           // colour model: viscosities of components A and B
+  float beta;               // colour model: recolouring parameter
   float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
   float isc[TSLB_NFIELDS];  // 1 / sc
 };
@@ -241,10 +243,9 @@ struct ArraySink {
 // f(+1) = m2 + m1 costs three operations per triple.
 template <bool ADD, typename Sink>
 __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, float Vz,
-                                            const Params &p, Sink &h) {
+                                            const Params &p, Sink &h, float om1) {
   const float drho = m[0], rho = 1.f + drho;
   const float ux = m[1], uy = m[2], uz = m[3];
-  const float om1 = p.om1;
   const float Bxx = om1 * m[4], Byy = om1 * m[5], Bzz = om1 * m[6];
   const float Bxy = om1 * m[7], Bxz = om1 * m[8], Byz = om1 * m[9];
   const float rVx = rho * Vx, rVy = rho * Vy, rVz = rho * Vz;
@@ -364,9 +365,9 @@ __device__ __forceinline__ float grad_axis(const Get &get) {
 // ---------------------------------------------------------------------------
 template <int NC>
 struct Src {
-  float m[NC == 2 ? 11 : 10];  // drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz(,phi)
-  float nn[3];                 // normals (two components)
-  float kap;                   // gated curvature (two components)
+  float m[NC >= 2 ? 11 : 10];  // drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz(,phi or psi)
+  float nn[3];                 // normals (phase field) or colour gradient (colour model)
+  float kap;                   // gated curvature (phase field)
 };
 
 // Raw buffer access.  A descriptor (gfx9 layout: 32-bit data format, no
@@ -434,17 +435,51 @@ template <int NC, typename S>
 __device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint32_t off4,
                                          const Params &p, Src<NC> &s) {
 #pragma unroll
-  for (int q = 0; q < (NC == 2 ? 11 : 10); q++) {
+  for (int q = 0; q < (NC >= 2 ? 11 : 10); q++) {
     const float v = to_f(bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS));
     if constexpr (sizeof(S) == 4) s.m[q] = v;
     else s.m[q] = v * p.isc[q];
   }
-  if constexpr (NC == 2) {
+  if constexpr (NC >= 2) {
     s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
     s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
     s.nn[2] = bld<float>(rsrc(ps.n + 2 * ps.fs4, ps.pb4), off4);
-    s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
   }
+  if constexpr (NC == 2) s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
+}
+
+// Colour-gradient perturbation added to each population on its way to the
+// sink (turbulent_benchmark/couette/recursive_colorTSLB3D.f90:764-956, in its
+// momentum-conserving quadratic form):
+//   Omega_l = K [w_l ((c_l.g)^2 - |g|^2) + (2/3) |g|^2 delta_l0],
+// K = A/(2|g|), A = (9/2) sigma omega, i.e. (5/27) A |g| at rest and
+// A ((c.g)^2 - |g|^2)/(k_l |g|) with k_l = 27, 108, 432; K = 0 below the
+// |g| > 0.001 threshold.
+template <typename Inner>
+struct ColourSink {
+  Inner &in;
+  float K, gx, gy, gz, g2;
+  template <int L>
+  __device__ __forceinline__ void put(float v) {
+    constexpr float cx = (float)kCX[L], cy = (float)kCY[L], cz = (float)kCZ[L];
+    const float cg = cx * gx + cy * gy + cz * gz;
+    float om = wgt(L) * (cg * cg - g2);
+    if constexpr (L == 0) om += (2.f / 3.f) * g2;
+    in.template put<L>(fmaf(K, om, v));
+  }
+  template <int G>
+  __device__ __forceinline__ void putg(float) {}
+};
+
+// colour model, second LDS pass: the A population of direction L from the
+// total one (shifted h, as stored) with Latva-Kokko recolouring,
+//   fA_l = cA (h_l + w_l) + bc w_l (c_l.g)/|c_l|,  bc = beta cA cB rho/|g|
+template <int L>
+__device__ __forceinline__ float colour_a(float h, float ca, float bc, float gx, float gy, float gz) {
+  constexpr int nnz = (kCX[L] != 0) + (kCY[L] != 0) + (kCZ[L] != 0);
+  constexpr float inv = nnz == 0 ? 0.f : (nnz == 1 ? 1.f : (nnz == 2 ? 0.70710678118654752f : 0.57735026918962576f));
+  const float cg = (float)kCX[L] * gx + (float)kCY[L] * gy + (float)kCZ[L] * gz;
+  return fmaf(ca, h + wgt(L), bc * (wgt(L) * inv) * cg);
 }
 
 template <int NC, bool ADD, typename Sink>
@@ -460,7 +495,7 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, Sink
     const float esx = sx + p.fx + p.lin_a * m[1];
     const float esy = sy + p.fy + p.lin_a * m[2];
     const float esz = sz + p.fz + p.lin_a * m[3];
-    all_pops_mt<false>(m, m[1] + esx, m[2] + esy, m[3] + esz, p, h);
+    all_pops_mt<false>(m, m[1] + esx, m[2] + esy, m[3] + esz, p, h, p.om1);
     // D3Q7 phase field with sharpening, recursive_2c_TSLB3D.f90:808-831
     const float p1g = 0.125f;
     const float sh = p1g * p.sharp_c * phi * (1.f - phi);
@@ -472,12 +507,32 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, Sink
     h.template putg<4>(base * (1.f - 3.f * m[2]) - sh * n1);
     gq[5] = base * (1.f + 3.f * m[3]) + sh * n2;
     gq[6] = base * (1.f - 3.f * m[3]) - sh * n2;
+  } else if constexpr (NC == 3) {
+    // colour model (recursive_colorTSLB3D.f90:719-959, corrected): per-node
+    // omega from the harmonic-mean viscosity of the two components (:756-758)
+    float ca = 0.5f * (1.f + m[10]);
+    ca = fminf(fmaxf(ca, 0.f), 1.f);
+    const float cb = 1.f - ca;
+    const float nu = 1.f / (ca / p.nua + cb / p.nub);
+    const float omega = 1.f / fmaf(3.f, nu, 0.5f);
+    const float gx = s.nn[0], gy = s.nn[1], gz = s.nn[2];
+    const float g2 = gx * gx + gy * gy + gz * gz;
+    const float mg = sqrtf(g2);
+    const bool pert = mg > 0.001f;
+    const float K = pert ? 2.25f * p.sigma * omega / mg : 0.f;
+    ColourSink<Sink> cs{h, K, gx, gy, gz, g2};
+    all_pops_mt<ADD>(m, m[1], m[2], m[3], p, cs, 1.f - omega);
+    gq[0] = ca;
+    gq[1] = pert ? p.beta * ca * cb * (1.f + m[0]) / mg : 0.f;
+    gq[2] = gx;
+    gq[3] = gy;
+    gq[4] = gz;
   } else {
     if (p.shift) {
       all_pops_mt<false>(m, m[1] + (p.fx + p.lin_a * m[1]), m[2] + (p.fy + p.lin_a * m[2]),
-                         m[3] + (p.fz + p.lin_a * m[3]), p, h);
+                         m[3] + (p.fz + p.lin_a * m[3]), p, h, p.om1);
     } else {
-      all_pops_mt<ADD>(m, m[1], m[2], m[3], p, h);
+      all_pops_mt<ADD>(m, m[1], m[2], m[3], p, h, p.om1);
     }
   }
 }
@@ -564,12 +619,57 @@ __device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int 
   acc_layer<CZ>(a, p);
 }
 
+// colour model: rewrite the 24 in-plane total populations of one LDS slot as
+// A populations (c = {cA, bc, gx, gy, gz})
+template <int L>
+__device__ __forceinline__ void colour_rewrite(float (*lds)[NSLOT], int slot, const float *c) {
+  if constexpr (L < 27) {
+    if constexpr (L != 5 && L != 6)
+      lds[qslot(L)][slot] = colour_a<L>(lds[qslot(L)][slot], c[0], c[1], c[2], c[3], c[4]);
+    colour_rewrite<L + 1>(lds, slot, c);
+  }
+}
+
+// colour model: A populations of the c_z = CZ layer arriving at (tx, ty)
+template <int CZ, bool MASKED>
+__device__ __forceinline__ float sum_layer(const float (*lds)[NSLOT], int tx, int ty, float own,
+                                           uint32_t bm) {
+  float s = 0.f;
+#pragma unroll
+  for (int cy = -1; cy <= 1; cy++)
+#pragma unroll
+    for (int cx = -1; cx <= 1; cx++) {
+      const int l = dir_of(cx, cy, CZ);
+      float v = (cx == 0 && cy == 0) ? own : lds[qslot(l)][(ty + 1 - cy) * SW + (tx + 1 - cx)];
+      if (MASKED && ((bm >> l) & 1u)) v = 0.f;
+      s += v;
+    }
+  return s;
+}
+
+// colour model: bounce-back of the A populations of the own node
+template <int CZ>
+__device__ __forceinline__ float bb_sum_layer(const float (*lds)[NSLOT], int tx, int ty, float a0,
+                                              float a5, float a6, uint32_t bm) {
+  float s = 0.f;
+  const int own = (ty + 1) * SW + (tx + 1);
+#pragma unroll
+  for (int cy = -1; cy <= 1; cy++)
+#pragma unroll
+    for (int cx = -1; cx <= 1; cx++) {
+      const int l = dir_of(cx, cy, CZ);
+      const int o = kOPP[l];
+      if ((bm >> l) & 1u) s += o == 0 ? a0 : (o == 5 ? a5 : (o == 6 ? a6 : lds[qslot(o)][own]));
+    }
+  return s;
+}
+
 template <int NC, bool MASK, bool ADD, int WPE, typename S>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
     const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
     const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, Params p) {
   constexpr int NQ = NC == 2 ? 28 : 24;
-  constexpr int NF = NC == 2 ? 11 : 10;
+  constexpr int NF = NC >= 2 ? 11 : 10;
   __shared__ float lds[NQ][NSLOT];
 
   const int tid = threadIdx.x;
@@ -647,14 +747,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     ook = zok && oxy;
     if (ook) {
       if constexpr (MASK) ook = (bld_u32(lk, o_xy * 4u) & kSolid) == 0u;
-      bool un = NC == 1;
+      bool un = NC != 2;  // flags gate the phase-field normals only
       if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)o_tile) != 0;
       if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), un ? o_xy * 4u : kOOB, p, o);
     }
     rok = zok && rxy;
     if (rok) {
       if constexpr (MASK) rok = (bld_u32(lk, r_xy * 4u) & kSolid) == 0u;
-      bool un = NC == 1;
+      bool un = NC != 2;  // flags gate the phase-field normals only
       if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)r_tile) != 0;
       if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), un ? r_xy * 4u : kOOB, p, r);
     }
@@ -670,6 +770,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     fetch(zs, co, co_ok, cr, cr_ok);
     float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
+    float oc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // colour: cA, bc, g
     // ---- own source slot
     if (co_ok) {
       PopSink sk{lds, (ty + 1) * SW + (tx + 1), 0.f, 0.f, 0.f};
@@ -682,11 +783,15 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         g5 = gq[5];
         g6 = gq[6];
       }
+      if constexpr (NC == 3)
+        for (int k = 0; k < 5; k++) oc[k] = gq[k];
     }
     // ---- halo ring slot
     if (cr_ok) {
       PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
       eval_src<NC, ADD>(cr, p, sk, gq);
+      if constexpr (NC == 3)
+        for (int k = 0; k < 5; k++) rc[k] = gq[k];
     }
     __syncthreads();
 
@@ -730,6 +835,34 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         }
       }
     }
+    if constexpr (NC == 3) {
+      // second LDS pass: every source slot turns its total populations into
+      // the recoloured A populations, and destinations sum those
+      __syncthreads();
+      float a0 = 0.f, a5 = 0.f, a6 = 0.f;
+      if (co_ok) {
+        colour_rewrite<1>(lds, (ty + 1) * SW + (tx + 1), oc);
+        a0 = colour_a<0>(h0, oc[0], oc[1], oc[2], oc[3], oc[4]);
+        a5 = colour_a<5>(h5, oc[0], oc[1], oc[2], oc[3], oc[4]);
+        a6 = colour_a<6>(h6, oc[0], oc[1], oc[2], oc[3], oc[4]);
+      }
+      if (cr_ok) colour_rewrite<1>(lds, rsy * SW + rsx, rc);
+      __syncthreads();
+      if (active) {
+        if ((bmA | bmB | bmC) == 0u) {
+          aA.phi += sum_layer<-1, false>(lds, tx, ty, a6, 0u);
+          aB.phi += sum_layer<0, false>(lds, tx, ty, a0, 0u);
+          aC.phi += sum_layer<1, false>(lds, tx, ty, a5, 0u);
+        } else {
+          aA.phi += sum_layer<-1, true>(lds, tx, ty, a6, bmA);
+          aB.phi += sum_layer<0, true>(lds, tx, ty, a0, bmB) +
+                    bb_sum_layer<-1>(lds, tx, ty, a0, a5, a6, bmB) +
+                    bb_sum_layer<0>(lds, tx, ty, a0, a5, a6, bmB) +
+                    bb_sum_layer<1>(lds, tx, ty, a0, a5, a6, bmB);
+          aC.phi += sum_layer<1, true>(lds, tx, ty, a5, bmC);
+        }
+      }
+    }
 
     // ---- finalize destination plane zs-1
     const int zd = zs - 1;
@@ -758,7 +891,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
                                aA.sxy - rho * u * v,
                                aA.sxz - rho * u * w,
                                aA.syz - rho * v * w,
-                               aA.phi};
+                               NC == 3 ? fmaf(2.f * aA.phi, inv, -1.f) : aA.phi};
 #pragma unroll
         for (int q = 0; q < NF; q++) {
           const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
@@ -790,7 +923,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 // interface gate phi(1-phi)|n|^2 > 0.2 (0 elsewhere, :614-631).  Across a
 // non-periodic face phi is read at the nearest interior node and n is 0; n is
 // 0 on solid nodes.  The phi plane three ahead is loaded into registers while
-// the current plane is computed.
+// the current plane is computed.  RAW (colour model) writes the unnormalised
+// gradient of the colour field psi and no curvature.
 //
 // Sparse output: n and kappa are zero wherever phi is locally uniform (the
 // bulk of both phases), so each (tile, plane) gets a flag byte, 1 when any of
@@ -801,7 +935,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
 static_assert(NPSLOT <= 2 * NTHR, "at most two phi footprint slots per thread");
 constexpr int NWAVE = NTHR / 64;
-template <typename S>
+template <typename S, bool RAW>
 __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ phi,
                                                          float *__restrict__ n0,
                                                          float *__restrict__ n1,
@@ -908,10 +1042,16 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
             return pl[c + dy * PW + dx];
           };
           const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
-          const float inv = 1.f / (sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f);
-          a = gx_ * inv;
-          bb = gy_ * inv;
-          cc = gz_ * inv;
+          if constexpr (RAW) {
+            a = gx_;
+            bb = gy_;
+            cc = gz_;
+          } else {
+            const float inv = 1.f / (sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f);
+            a = gx_ * inv;
+            bb = gy_ * inv;
+            cc = gz_ * inv;
+          }
         }
       }
       dn[0][sl] = a;
@@ -965,7 +1105,7 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
       c = nl[z0s][2][own];
       const float f = ph[z0s][(ty + 2) * PW + (tx + 2)];
       const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
-      if (ind > 0.2f) {
+      if (!RAW && ind > 0.2f) {
         auto nat = [&](int comp) {
           return [&, comp](int dx, int dy, int dz) -> float {
             const int zz = dz < 0 ? zm : (dz > 0 ? zp : z0s);
@@ -1108,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_zplane_bc(FieldsT<S> f, const float *__
   if (id >= plane) return;
   const int lo = id, lo1 = id + plane;
   const int hi = id + plane * (p.nz - 1), hi1 = id + plane * (p.nz - 2);
-  const int nf = ncomp == 2 ? TSLB_NFIELDS : 10;
+  const int nf = ncomp >= 2 ? TSLB_NFIELDS : 10;
   for (int q = 0; q < nf; q++) f.f[q][hi] = f.f[q][hi1];
   for (int q = 0; q < nf; q++) {
     if (q == TSLB_U || q == TSLB_V) stf(f.f[q], lo, 0.f, p, q);
@@ -1198,7 +1338,7 @@ __global__ __launch_bounds__(256) void k_jet_bc(const S *__restrict__ src, S *__
     const float ux = at(TSLB_U, x), vx = at(TSLB_V, x);
     const float wx = side ? at(TSLB_W, xi + p.nx * xj + plane * zy) : at(TSLB_W, x);
     ArraySink bnd;
-    all_pops_mt<false>(sy.m, ux, vx, wx, p, bnd);
+    all_pops_mt<false>(sy.m, ux, vx, wx, p, bnd, p.om1);
     const float d = bnd.h[l] - own.h[kOPP[l]];
     const float cx = (float)kCX[l], cy = (float)kCY[l], cz = (float)kCZ[l];
     sm[0] += d;
@@ -1325,6 +1465,7 @@ __global__ __launch_bounds__(256) void k_reduce_part(CFieldsT<S> f, const uint32
     const double u = ldf(f.f[1], id, p, 1), v = ldf(f.f[2], id, p, 2), w = ldf(f.f[3], id, p, 3);
     acc[0] += rho;
     if (p.ncomp == 2) acc[1] += (double)ldf(f.f[TSLB_PHI], id, p, TSLB_PHI);
+    if (p.ncomp == TSLB_COLOUR) acc[1] += 0.5 * rho * (1.0 + (double)ldf(f.f[TSLB_PHI], id, p, TSLB_PHI));
     acc[2] += 0.5 * rho * (u * u + v * v + w * w);
     acc[3] += rho * u;
     acc[4] += rho * v;
@@ -1502,10 +1643,16 @@ int launch_normals(tslb_ctx *c, int dense = 0) {
   const int nblk = c->kp.tiles_x * c->kp.tiles_y * c->kp.nzc;
   with_storage(c, [&](auto tag) {
     using S = decltype(tag);
-    hipLaunchKernelGGL(k_normals_tiled<S>, dim3(nblk), dim3(NTHR), 0, c->stream,
-                       field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
-                       c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->flags, c->links, c->has_mask,
-                       dense, c->kp);
+    if (c->prm.ncomp == TSLB_COLOUR)
+      hipLaunchKernelGGL((k_normals_tiled<S, true>), dim3(nblk), dim3(NTHR), 0, c->stream,
+                         field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
+                         c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->flags, c->links, c->has_mask, 1,
+                         c->kp);
+    else
+      hipLaunchKernelGGL((k_normals_tiled<S, false>), dim3(nblk), dim3(NTHR), 0, c->stream,
+                         field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
+                         c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->flags, c->links, c->has_mask,
+                         dense, c->kp);
     return 0;
   });
   HIPCK(hipGetLastError());
@@ -1526,8 +1673,7 @@ int launch_fused(tslb_ctx *c) {
     e1 = take_event(c);
     HIPCK(hipEventRecord(e0, c->stream));
   }
-  const bool add = p.ncomp == 1 && !p.shift &&
-                   (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
+  const bool add = p.ncomp != 2 && !p.shift && (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
   const bool w4 = (c->variant & 2) != 0;
   with_storage(c, [&](auto tag) {
     using S = decltype(tag);
@@ -1550,6 +1696,14 @@ int launch_fused(tslb_ctx *c) {
     if (p.ncomp == 2) {
       if (c->has_mask) TSLB_LAUNCH_V(2, true, false);
       else TSLB_LAUNCH_V(2, false, false);
+    } else if (p.ncomp == TSLB_COLOUR) {
+      if (c->has_mask) {
+        if (add) TSLB_LAUNCH_V(3, true, true);
+        else TSLB_LAUNCH_V(3, true, false);
+      } else {
+        if (add) TSLB_LAUNCH_V(3, false, true);
+        else TSLB_LAUNCH_V(3, false, false);
+      }
     } else if (add) {
       if (c->has_mask) TSLB_LAUNCH_V(1, true, true);
       else TSLB_LAUNCH_V(1, false, true);
@@ -1633,6 +1787,7 @@ void tslb_default_params(tslb_params *p) {
   p->storage = TSLB_STORAGE_FP32;
   p->device = 0;
   p->tau = 0.8f;
+  p->beta = 0.7f;
 }
 
 int tslb_create(tslb_ctx **out, const tslb_params *p) {
@@ -1645,7 +1800,10 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   // kernels address one z-plane of a field through a buffer descriptor with
   // 32-bit byte offsets
   if ((long long)p->nx * p->ny * 4 > 0x7fffffffLL) return fail(TSLB_E_ARG, "nx*ny*4 bytes must fit in int32");
-  if (p->ncomp != 1 && p->ncomp != 2) return fail(TSLB_E_ARG, "ncomp must be 1 or 2");
+  if (p->ncomp != 1 && p->ncomp != 2 && p->ncomp != TSLB_COLOUR)
+    return fail(TSLB_E_ARG, "ncomp must be 1, 2 or TSLB_COLOUR");
+  if (p->ncomp == TSLB_COLOUR && p->tau_b > 0.f && !(p->tau_b > 0.5f))
+    return fail(TSLB_E_ARG, "tau_b must be > 0.5 (or <= 0 for tau)");
   if (!(p->tau > 0.5f)) return fail(TSLB_E_ARG, "tau must be > 0.5");
   if (p->storage != TSLB_STORAGE_FP32 && p->storage != TSLB_STORAGE_FP16 &&
       p->storage != TSLB_STORAGE_BF16)
@@ -1670,7 +1828,11 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.py = p->periodic[1] ? 1 : 0;
   k.pz = p->periodic[2] ? 1 : 0;
   k.ncomp = p->ncomp;
-  k.shift = (p->ncomp == 2 || p->force_mode == TSLB_FORCE_SHIFT) ? 1 : 0;
+  // the colour model takes its body force additively (recursive_colorTSLB3D.f90's fx*p1dcssq)
+  k.shift = (p->ncomp == 2 || (p->ncomp == 1 && p->force_mode == TSLB_FORCE_SHIFT)) ? 1 : 0;
+  k.nua = (p->tau - 0.5f) / 3.f;
+  k.nub = ((p->tau_b > 0.5f ? p->tau_b : p->tau) - 0.5f) / 3.f;
+  k.beta = p->beta;
   k.om1 = 1.f - 1.f / p->tau;
   k.sigma = p->sigma;
   k.sharp_c = p->sharp_c;
@@ -1718,8 +1880,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   }
   const size_t fb = c->fs * TSLB_NFIELDS * c->es;
   if ((rc = dev_alloc(c, (void **)&c->buf[0], fb)) || (rc = dev_alloc(c, (void **)&c->buf[1], fb)) ||
-      (p->ncomp == 2 && (rc = dev_alloc(c, (void **)&c->nrm, 4 * c->fs * sizeof(float)))) ||
-      (p->ncomp == 2 &&
+      (p->ncomp >= 2 && (rc = dev_alloc(c, (void **)&c->nrm, 4 * c->fs * sizeof(float)))) ||
+      (p->ncomp >= 2 &&
        (rc = dev_alloc(c, (void **)&c->flags, (size_t)k.tiles_x * k.tiles_y * p->nz))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
       (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
@@ -1948,7 +2110,8 @@ int tslb_download_normals(tslb_ctx *c, float *n3) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (!n3) return fail(TSLB_E_ARG, "null pointer");
-  if (c->prm.ncomp != 2) return fail(TSLB_E_STATE, "normals exist only for two-component runs");
+  if (c->prm.ncomp < 2)
+    return fail(TSLB_E_STATE, "normals (colour model: colour gradients) exist only for two-component runs");
   // the stepping pass writes n only where it is nonzero; recompute densely
   if (c->normals_fresh != 2 && (rc = launch_normals(c, 1))) return rc;
   for (int a = 0; a < 3; a++)
@@ -1986,14 +2149,14 @@ int tslb_init_equilibrium(tslb_ctx *c, int prepush) {
 int tslb_normals(tslb_ctx *c) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (c->prm.ncomp != 2) return TSLB_OK;
+  if (c->prm.ncomp < 2) return TSLB_OK;
   return launch_normals(c);
 }
 
 int tslb_collide_stream(tslb_ctx *c) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (c->prm.ncomp == 2 && !c->normals_fresh) return fail(TSLB_E_STATE, "tslb_normals() must precede tslb_collide_stream()");
+  if (c->prm.ncomp >= 2 && !c->normals_fresh) return fail(TSLB_E_STATE, "tslb_normals() must precede tslb_collide_stream()");
   return launch_fused(c);
 }
 
@@ -2002,7 +2165,7 @@ int tslb_step(tslb_ctx *c, int nsteps) {
   if (rc) return rc;
   if (nsteps < 0) return fail(TSLB_E_ARG, "nsteps < 0");
   for (int s = 0; s < nsteps; s++) {
-    if (c->prm.ncomp == 2 && (rc = launch_normals(c))) return rc;
+    if (c->prm.ncomp >= 2 && (rc = launch_normals(c))) return rc;
     if ((rc = launch_fused(c))) return rc;
     if ((rc = launch_zbc(c))) return rc;
   }
@@ -2022,6 +2185,7 @@ int tslb_set_jet_bc(tslb_ctx *c, int enable) {
   if (!c->has_mask || c->zwalls != 3)
     return fail(TSLB_E_STATE, "jet boundary needs every node of planes k=1 and k=nz solid (tslb_set_isfluid)");
   if (c->zbc) return fail(TSLB_E_STATE, "jet boundary and tslb_set_zplane_bc are exclusive");
+  if (c->prm.ncomp == TSLB_COLOUR) return fail(TSLB_E_STATE, "the jet boundary is defined for the phase-field model");
   c->jet = 1;
   c->kp.gwall = 1;
   return TSLB_OK;

@@ -11,6 +11,7 @@ module tslb_mod
   integer(c_int), parameter :: TSLB_PXX = 4, TSLB_PYY = 5, TSLB_PZZ = 6
   integer(c_int), parameter :: TSLB_PXY = 7, TSLB_PXZ = 8, TSLB_PYZ = 9, TSLB_PHI = 10
   integer(c_int), parameter :: TSLB_FORCE_ADDITIVE = 0, TSLB_FORCE_SHIFT = 1
+  integer(c_int), parameter :: TSLB_COLOUR = 3
   integer(c_int), parameter :: TSLB_STORAGE_FP32 = 0, TSLB_STORAGE_BF16 = 1, TSLB_STORAGE_FP16 = 2
 
   type, bind(C) :: tslb_params
@@ -25,6 +26,8 @@ module tslb_mod
     real(c_float) :: sharp_c
     real(c_float) :: force(3)
     real(c_float) :: lin_force
+    real(c_float) :: tau_b
+    real(c_float) :: beta
   end type tslb_params
 
   interface

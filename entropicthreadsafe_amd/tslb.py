@@ -42,6 +42,8 @@ class Params(ctypes.Structure):
         ("sharp_c", ctypes.c_float),
         ("force", ctypes.c_float * 3),
         ("lin_force", ctypes.c_float),
+        ("tau_b", ctypes.c_float),
+        ("beta", ctypes.c_float),
     ]
 
 
@@ -139,7 +141,7 @@ class Lattice:
     """
 
     def __init__(self, nx, ny, nz, ncomp=1, tau=0.8, sigma=0.0, sharp_c=0.0,
-                 force=(0.0, 0.0, 0.0), force_mode=FORCE_ADDITIVE, lin_force=0.0,
+                 force=(0.0, 0.0, 0.0), force_mode=FORCE_ADDITIVE, lin_force=0.0, tau_b=0.0, beta=0.7,
                  periodic=(1, 1, 1), storage="fp32", device=0):
         self.lib = load_library()
         p = Params()
@@ -156,6 +158,8 @@ class Lattice:
         p.sigma = float(sigma)
         p.sharp_c = float(sharp_c)
         p.lin_force = float(lin_force)
+        p.tau_b = float(tau_b)
+        p.beta = float(beta)
         self.params = p
         self.shape = (int(nz), int(ny), int(nx))
         self.n = int(nx) * int(ny) * int(nz)

@@ -38,6 +38,11 @@ extern "C" {
 #define TSLB_FORCE_ADDITIVE 0 /* f_l += w_l c_l.F / cs^2  (recursive_TSLB3D.f90:571-707) */
 #define TSLB_FORCE_SHIFT 1    /* u_eq = u + F            (lb_kernels_module.f90:585-587)  */
 
+/* ncomp value of the colour-gradient model (turbulent_benchmark/couette/
+ * recursive_colorTSLB3D.f90:719-959): field TSLB_PHI then holds the colour
+ * psi = (rhoA - rhoB)/rho, sigma is the surface tension. */
+#define TSLB_COLOUR 3
+
 #define TSLB_STORAGE_FP32 0
 #define TSLB_STORAGE_BF16 1
 #define TSLB_STORAGE_FP16 2
@@ -60,7 +65,10 @@ extern "C" {
  * (recursive_2c_TSLB3D.f90:20-52, recursive_TSLB3D.f90:20-48). */
 typedef struct tslb_params {
   int32_t nx, ny, nz;    /* lattice extent (physical nodes, no ghosts)            */
-  int32_t ncomp;         /* 1: single component; 2: colour-gradient two-component  */
+  int32_t ncomp;         /* 1: single component; 2: two components, phase field with
+                            sharpening and CSF (the reference's TSLB 2C path);
+                            TSLB_COLOUR: two components, colour gradient with
+                            perturbation and recolouring                           */
   int32_t periodic[3];   /* 1: periodic axis; 0: halfway bounce-back wall just
                             outside both faces of that axis                         */
   int32_t force_mode;    /* TSLB_FORCE_*; two-component runs always shift          */
@@ -71,6 +79,9 @@ typedef struct tslb_params {
   float sharp_c;         /* phase-field sharpening coefficient, :28                  */
   float force[3];        /* constant body force fx, fy, fz                           */
   float lin_force;       /* linear forcing coefficient A: u_eq += A u (shift mode)   */
+  float tau_b;           /* colour model: relaxation time of component B (<= 0.5:
+                            tau); per node omega from the harmonic-mean viscosity   */
+  float beta;            /* colour model: recolouring parameter (0..1)               */
 } tslb_params;
 
 typedef struct tslb_ctx tslb_ctx;

@@ -51,6 +51,10 @@ def lib():
             getattr(L, f"orc_feq_{sfx}").argtypes = [ct, ct, ct, ct, cp]
             getattr(L, f"orc_aneq_{sfx}").argtypes = [cp, ct, ct, ct, cp]
             getattr(L, f"orc_set_jet_{sfx}").argtypes = [P, ctypes.c_int]
+            getattr(L, f"orc_set_colour_{sfx}").argtypes = [P, ctypes.c_double, ctypes.c_double]
+            getattr(L, f"orc_colour_values_{sfx}").argtypes = [
+                ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ct, cp, cp, ct, cp,
+                cp, cp]
             getattr(L, f"orc_jet_values_{sfx}").argtypes = [ct, ct, cp, cp, cp, cp]
         L.orc_rand_noseeded.restype = ctypes.c_float
         L.orc_rand_noseeded.argtypes = [ctypes.c_int] * 4
@@ -105,6 +109,20 @@ def jet_values(omega, rho_y, P_y, u_y, u_x, dtype=np.float64):
     return out
 
 
+def colour_values(omega_a, omega_b, sigma, beta, rho, u3, P6, psi, g3, dtype=np.float64):
+    """Total and component-A post-collision populations of the colour-gradient
+    model for one node (orc_colour_values)."""
+    sfx, ct = ("d", ctypes.c_double) if dtype == np.float64 else ("f", ctypes.c_float)
+    u = np.ascontiguousarray(u3, dtype=dtype)
+    P = np.ascontiguousarray(P6, dtype=dtype)
+    g = np.ascontiguousarray(g3, dtype=dtype)
+    fo = np.zeros(27, dtype=dtype)
+    fa = np.zeros(27, dtype=dtype)
+    getattr(lib(), f"orc_colour_values_{sfx}")(omega_a, omega_b, sigma, beta, rho, _ptr(u, ct), _ptr(P, ct),
+                                               psi, _ptr(g, ct), _ptr(fo, ct), _ptr(fa, ct))
+    return fo, fa
+
+
 class Oracle:
     """One oracle lattice (population-push form, plus the moment-space pull form).
 
@@ -145,6 +163,10 @@ class Oracle:
         a = np.ascontiguousarray(isf, dtype=np.int32).reshape(-1)
         assert a.size == self.n
         self._fn("set_isfluid")(self._h, _ptr(a, ctypes.c_int32))
+
+    def set_colour(self, tau_b, beta):
+        """Colour-gradient model (ncomp=3): tau of component B and recolouring beta."""
+        self._fn("set_colour")(self._h, float(tau_b), float(beta))
 
     def set_jet(self, enable=True):
         """Turbulent-jet inlet (k=0) / outlet (k=nz-1) wall planes; set before init."""

@@ -19,6 +19,8 @@
  *     1C/2C velocity-shift     TSLB_2C-mpi/lb_kernels_module.f90:585-587,769-771
  *   - bounce-back on solids    boundary_cds_3D_module.f90:10-75
  *   - jet inlet / outlet       boundary_cds_3D_module.f90:862-1049
+ *   - colour-gradient model    turbulent_benchmark/couette/recursive_colorTSLB3D.f90:
+ *                              719-959 (restated with its defects corrected)
  *   - periodic ghost wrap      TSLB_newBCS/boundary_cds_3D_module.f90:1294-1877
  *   - counter-based RNG        TSLB_newBCS/vars_single_3D_module.f90:78-165
  *   - initial state (f=feq, g pre-pushed)  recursive_2c_TSLB3D.f90:106-287
@@ -82,6 +84,9 @@ enum { F_RHO = 0, F_U, F_V, F_W, F_PXX, F_PYY, F_PZZ, F_PXY, F_PXZ, F_PYZ, F_PHI
 typedef struct {
   int nx, ny, nz, ncomp, per[3], force_mode;
   int jet;            /* turbulent-jet inlet/outlet planes (orc_set_jet) */
+  real tau_b, beta;   /* colour-gradient model (ncomp 3): tau of B, recolouring */
+  real *fa;           /* colour model: component-A populations [27][ng] */
+  real *apops;        /* colour model pull scratch: [27][n] */
   real omega, sigma, sharp_c, fx, fy, fz, lin_a;
   size_t n, ng;
   real *f, *g;        /* [27][ng], [7][ng] */
@@ -260,6 +265,9 @@ void *FN(orc_create)(int nx, int ny, int nz, int ncomp, const int *periodic, dou
   c->ng = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
   c->f = (real *)calloc(27 * c->ng, sizeof(real));
   c->g = (real *)calloc(7 * c->ng, sizeof(real));
+  c->tau_b = (real)tau;
+  c->beta = (real)0.7;
+  c->fa = ncomp == 3 ? (real *)calloc(27 * c->ng, sizeof(real)) : NULL;
   for (int q = 0; q < NFIELD; q++) c->m[q] = (real *)calloc(c->n, sizeof(real));
   for (int a = 0; a < 3; a++) {
     c->nrm[a] = (real *)calloc(c->n, sizeof(real));
@@ -285,6 +293,8 @@ void FN(orc_destroy)(void *h) {
   free(c->isf);
   free(c->pops);
   free(c->gpops);
+  free(c->fa);
+  free(c->apops);
   free(c);
 }
 
@@ -365,6 +375,11 @@ static void k1_moments(ctx_t *c) {
           real phi = 0;
           for (int l = 0; l < 7; l++) phi += c->g[(size_t)l * c->ng + gp];
           c->m[F_PHI][p] = phi;
+        }
+        if (c->ncomp == 3) {
+          real ra = 0;
+          for (int l = 0; l < 27; l++) ra += c->fa[(size_t)l * c->ng + gp];
+          c->m[F_PHI][p] = (real)2 * ra / rho - (real)1;
         }
       }
 }
@@ -560,6 +575,106 @@ static void jet_bc_push(ctx_t *c) {
     }
 }
 
+/* ---------------------------------------------------------------------- */
+/* Colour-gradient model (ncomp 3): two components A and B carried as the   */
+/* total populations f = fA + fB and the A populations fA, with the colour  */
+/* field psi = (rhoA - rhoB)/rho stored in place of phi.  Formula source:   */
+/* turbulent_benchmark/couette/recursive_colorTSLB3D.f90:719-959, which     */
+/* does not compile; restated with its defects corrected:                   */
+/*  - psi = (rhoA - rhoB)/rho (the file inverts it, :511);                  */
+/*  - u from the summed momenta over rho = rhoA + rhoB (:494-508);          */
+/*  - the perturbation is quadratic in c.grad(psi) (the file's linear form  */
+/*    breaks momentum conservation, :772 ff):                               */
+/*      Omega_0 = (5/27) A |g|,  Omega_l = A ((c_l.g)^2 - |g|^2)/(k_l |g|),  */
+/*      k_l = 27, 108, 432 for |c_l|^2 = 1, 2, 3 (the file's coefficients,  */
+/*    :764-956), which conserves mass and momentum and gives the interface  */
+/*    stress (A/(9|g|))(g g - |g|^2 I); its surface tension is 2A/(9 omega), */
+/*    so A = (9/2) sigma omega makes sigma the surface tension (the file's   */
+/*    st_coeff = (9/4) sigma omega, :760, would give sigma/2);               */
+/*  - Latva-Kokko recolouring with cos = c.g/(|c||g|) (the file divides by  */
+/*    |g|^2, :773):  fA_l = (rhoA/rho) f_l + beta (rhoA rhoB/rho^2) w_l rho  */
+/*    (c_l.g)/(|c_l||g|), both only where |g| > 0.001 (:763);              */
+/*  - per-node omega from the harmonic mean viscosity (:756-758) of         */
+/*    nu_A = (tau - 1/2)/3 and nu_B = (tau_b - 1/2)/3;                       */
+/*  - additive body force w_l (F.c_l)/cs^2 (the file's fx*p1dcssq).          */
+/* The gradient g = (1/cs^2) sum_l w_l c_l psi(x + c_l) is the stencil of   */
+/* grad_stencil.                                                             */
+/* ---------------------------------------------------------------------- */
+void FN(orc_set_colour)(void *h, double tau_b, double beta) {
+  ctx_t *c = (ctx_t *)h;
+  c->tau_b = (real)tau_b;
+  c->beta = (real)beta;
+}
+
+static void k2_colour_grad(ctx_t *c) {
+  const int nx = c->nx, ny = c->ny, nz = c->nz;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        size_t p = idx(c, i, j, k);
+        if (!c->isf[p]) {
+          c->nrm[0][p] = c->nrm[1][p] = c->nrm[2][p] = 0;
+          continue;
+        }
+        real g3[3];
+        grad_stencil(c, i, j, k, -1, g3);
+        for (int a = 0; a < 3; a++) c->nrm[a][p] = g3[a];
+      }
+}
+
+/* total and A post-collision populations of one fluid node from its moments
+ * (rho, u, Pi, psi) and colour gradient g */
+static void colour_values(real rho, real u, real v, real w, const real *P, real psi, const real *g3,
+                          real omega_a, real omega_b, real sigma, real beta, real fx, real fy, real fz,
+                          real *fo, real *fao) {
+  real ca = (real)0.5 * ((real)1 + psi);
+  if (ca < 0) ca = 0;
+  if (ca > 1) ca = 1;
+  const real cb = (real)1 - ca;
+  const real nua = ((real)1 / omega_a - (real)0.5) / (real)3;
+  const real nub = ((real)1 / omega_b - (real)0.5) / (real)3;
+  const real nu = (real)1 / (ca / nua + cb / nub);
+  const real omega = (real)1 / ((real)3 * nu + (real)0.5);
+  const real mg2 = g3[0] * g3[0] + g3[1] * g3[1] + g3[2] * g3[2];
+  const real mg = sizeof(real) == 4 ? (real)sqrtf((float)mg2) : (real)sqrt((double)mg2);
+  const int pert = mg > (real)0.001;
+  const real A = (real)4.5 * sigma * omega;
+  for (int l = 0; l < 27; l++) {
+    real val = feq_l(l, rho, u, v, w) + ((real)1 - omega) * aneq_l(l, P, u, v, w) * wgt(l);
+    if (l > 0) val += (fx * (real)CX[l] + fy * (real)CY[l] + fz * (real)CZ[l]) * (wgt(l) * (real)3);
+    real om2 = 0;
+    if (pert) {
+      if (l == 0) {
+        val += (real)5 * A * mg / (real)27;
+      } else {
+        const int nnz = (CX[l] != 0) + (CY[l] != 0) + (CZ[l] != 0);
+        const real k = nnz == 1 ? (real)27 : (nnz == 2 ? (real)108 : (real)432);
+        const real cg = (real)CX[l] * g3[0] + (real)CY[l] * g3[1] + (real)CZ[l] * g3[2];
+        val += A * (cg * cg - mg2) / (k * mg);
+        const real cl = nnz == 1 ? (real)1 : (nnz == 2 ? (real)1.4142135623730951 : (real)1.7320508075688772);
+        om2 = beta * ca * cb * wgt(l) * rho * cg / (cl * mg);
+      }
+    }
+    fo[l] = val;
+    fao[l] = ca * val + om2;
+  }
+}
+
+void FN(orc_colour_values)(double omega_a, double omega_b, double sigma, double beta, real rho,
+                           const real *u3, const real *P6, real psi, const real *g3, real *fo,
+                           real *fao) {
+  colour_values(rho, u3[0], u3[1], u3[2], P6, psi, g3, (real)omega_a, (real)omega_b, (real)sigma,
+                (real)beta, 0, 0, 0, fo, fao);
+}
+
+static void colour_node(const ctx_t *c, size_t p, real *fo, real *fao) {
+  real P[6], g3[3] = {c->nrm[0][p], c->nrm[1][p], c->nrm[2][p]};
+  for (int q = 0; q < 6; q++) P[q] = c->m[F_PXX + q][p];
+  colour_values(c->m[F_RHO][p], c->m[F_U][p], c->m[F_V][p], c->m[F_W][p], P, c->m[F_PHI][p], g3,
+                c->omega, (real)1 / c->tau_b, c->sigma, c->beta, c->fx, c->fy, c->fz, fo, fao);
+}
+
 /* K3 push on fluid nodes, then boundary resolve */
 static void k3_push(ctx_t *c) {
   const int nx = c->nx, ny = c->ny, nz = c->nz;
@@ -570,6 +685,16 @@ static void k3_push(ctx_t *c) {
         size_t p = idx(c, i, j, k);
         if (!c->isf[p]) continue;
         real s3[3] = {0, 0, 0}, fo[27], go[7];
+        if (c->ncomp == 3) {
+          real fao[27];
+          colour_node(c, p, fo, fao);
+          for (int l = 0; l < 27; l++) {
+            const size_t d = (size_t)l * c->ng + gidx(c, i + CX[l], j + CY[l], k + CZ[l]);
+            c->f[d] = fo[l];
+            c->fa[d] = fao[l];
+          }
+          continue;
+        }
         if (c->ncomp == 2) csf_shift(c, i, j, k, s3);
         collide_node(c, p, s3, fo, go);
         for (int l = 0; l < 27; l++)
@@ -613,9 +738,11 @@ void FN(orc_step)(void *h, int nsteps) {
   ctx_t *c = (ctx_t *)h;
   for (int s = 0; s < nsteps; s++) {
     if (c->ncomp == 2) k2_normals(c);
+    if (c->ncomp == 3) k2_colour_grad(c);
     k3_push(c);
     resolve(c, c->f, 27);
     if (c->ncomp == 2) resolve(c, c->g, 7);
+    if (c->ncomp == 3) resolve(c, c->fa, 27);
     if (c->jet) jet_bc_push(c);
     k1_moments(c);
   }
@@ -638,6 +765,7 @@ void FN(orc_init)(void *h, const real *rho, const real *u, const real *v, const 
   }
   memset(c->f, 0, 27 * c->ng * sizeof(real));
   memset(c->g, 0, 7 * c->ng * sizeof(real));
+  if (c->fa) memset(c->fa, 0, 27 * c->ng * sizeof(real));
 #pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; k++)
     for (int j = 0; j < ny; j++)
@@ -647,6 +775,11 @@ void FN(orc_init)(void *h, const real *rho, const real *u, const real *v, const 
         size_t gp = gidx(c, i, j, k);
         for (int l = 0; l < 27; l++)
           c->f[(size_t)l * c->ng + gp] = feq_l(l, c->m[F_RHO][p], c->m[F_U][p], c->m[F_V][p], c->m[F_W][p]);
+        if (c->ncomp == 3) {
+          const real ca = (real)0.5 * ((real)1 + c->m[F_PHI][p]);
+          for (int l = 0; l < 27; l++)
+            c->fa[(size_t)l * c->ng + gp] = ca * c->f[(size_t)l * c->ng + gp];
+        }
         if (c->ncomp == 2) {
           real ph = c->m[F_PHI][p];
           real u3[3] = {c->m[F_U][p], c->m[F_V][p], c->m[F_W][p]};
@@ -681,10 +814,12 @@ void FN(orc_pull_step)(void *h, int nsteps) {
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   if (!c->pops) c->pops = (real *)malloc(27 * c->n * sizeof(real));
   if (!c->gpops) c->gpops = (real *)malloc(7 * c->n * sizeof(real));
+  if (c->ncomp == 3 && !c->apops) c->apops = (real *)malloc(27 * c->n * sizeof(real));
   real *nm[NFIELD];
   for (int q = 0; q < NFIELD; q++) nm[q] = (real *)malloc(c->n * sizeof(real));
   for (int s = 0; s < nsteps; s++) {
     if (c->ncomp == 2) k2_normals(c);
+    if (c->ncomp == 3) k2_colour_grad(c);
     /* post-collision populations at every fluid source, node-local */
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < nz; k++)
@@ -693,6 +828,15 @@ void FN(orc_pull_step)(void *h, int nsteps) {
           size_t p = idx(c, i, j, k);
           if (!c->isf[p]) continue;
           real s3[3] = {0, 0, 0}, fo[27], go[7];
+          if (c->ncomp == 3) {
+            real fao[27];
+            colour_node(c, p, fo, fao);
+            for (int l = 0; l < 27; l++) {
+              c->pops[(size_t)l * c->n + p] = fo[l];
+              c->apops[(size_t)l * c->n + p] = fao[l];
+            }
+            continue;
+          }
           if (c->ncomp == 2) csf_shift(c, i, j, k, s3);
           collide_node(c, p, s3, fo, go);
           for (int l = 0; l < 27; l++) c->pops[(size_t)l * c->n + p] = fo[l];
@@ -709,12 +853,13 @@ void FN(orc_pull_step)(void *h, int nsteps) {
             for (int q = 0; q < NFIELD; q++) nm[q][p] = c->m[q][p];
             continue;
           }
-          real fl[27], gl[7], P[6], rho, u, v, w;
+          real fl[27], gl[7], al[27], P[6], rho, u, v, w;
           for (int l = 0; l < 27; l++) {
             int si = i - CX[l], sj = j - CY[l], sk = k - CZ[l];
             int inside = wrap_src(c, &si, &sj, &sk);
             if (l > 0 && (!inside || !c->isf[idx(c, si, sj, sk)])) {
               fl[l] = c->pops[(size_t)OPP[l] * c->n + p];
+              if (c->ncomp == 3) al[l] = c->apops[(size_t)OPP[l] * c->n + p];
               if (l < 7 && c->ncomp == 2)
                 gl[l] = c->jet ? (real)0 : c->gpops[(size_t)OPP[l] * c->n + p];
               /* jet inlet / outlet: the source column x on the wall plane */
@@ -733,6 +878,7 @@ void FN(orc_pull_step)(void *h, int nsteps) {
             } else {
               size_t sp = idx(c, si, sj, sk);
               fl[l] = c->pops[(size_t)l * c->n + sp];
+              if (c->ncomp == 3) al[l] = c->apops[(size_t)l * c->n + sp];
               if (l < 7 && c->ncomp == 2) gl[l] = c->gpops[(size_t)l * c->n + sp];
             }
           }
@@ -746,6 +892,10 @@ void FN(orc_pull_step)(void *h, int nsteps) {
             real phi = 0;
             for (int l = 0; l < 7; l++) phi += gl[l];
             nm[F_PHI][p] = phi;
+          } else if (c->ncomp == 3) {
+            real ra = 0;
+            for (int l = 0; l < 27; l++) ra += al[l];
+            nm[F_PHI][p] = (real)2 * ra / rho - (real)1;
           } else {
             nm[F_PHI][p] = c->m[F_PHI][p];
           }
@@ -784,7 +934,7 @@ void FN(orc_reduce)(void *h, double *out) {
         if (!c->isf[p]) continue;
         double r = c->m[F_RHO][p], u = c->m[F_U][p], v = c->m[F_V][p], w = c->m[F_W][p];
         s[0] += r;
-        s[1] += (double)c->m[F_PHI][p];
+        s[1] += c->ncomp == 3 ? 0.5 * r * (1.0 + (double)c->m[F_PHI][p]) : (double)c->m[F_PHI][p];
         s[2] += 0.5 * r * (u * u + v * v + w * w);
         s[3] += r * u;
         s[4] += r * v;

@@ -34,8 +34,8 @@ def test_version_and_last_error():
 
 
 def test_params_layout_matches_header():
-    # 10 int32 + 7 float32 = 68 bytes, no padding
-    assert tslb.ctypes.sizeof(tslb.Params) == 68
+    # 10 int32 + 9 float32 = 76 bytes, no padding
+    assert tslb.ctypes.sizeof(tslb.Params) == 76
     p = tslb.Params()
     tslb.load_library().tslb_default_params(tslb.ctypes.byref(p))
     assert (p.nx, p.ny, p.nz, p.ncomp) == (16, 16, 16, 1)
