@@ -19,10 +19,12 @@
 //   k_fused     LDS-tiled 2.5D collide + pull-stream.  A 64x8 block marches a
 //               z-chunk; per source plane it evaluates the 27 (+7) populations
 //               of every node of the (64+2)x(8+2) footprint once, scatters the
-//               24 (+4) in-plane-moving ones through LDS, and each destination
-//               column gathers its populations and accumulates moment sums for
-//               three destination planes in registers.
-//   k_normals   colour gradient / normals (two-component runs).
+//               24 in-plane-moving ones (+2 phase fluxes) through LDS, and each
+//               destination column gathers its populations and accumulates
+//               moment sums for three destination planes in registers.  Phase
+//               field: the normals come from three phi planes kept in an LDS
+//               ring.
+//   k_normals   colour gradient (colour model) or sparse curvature (phase field).
 //   k_prepush   initial one-step advection of phi (reference writes g pushed).
 //   k_links     per-node bounce-back link masks from an isfluid map.
 //   k_zplane_bc moment-space inlet/outlet planes.
@@ -97,6 +99,9 @@ constexpr int SW = TX + 2, SH = TY + 2, NSLOT = SW * SH;
 constexpr int NRING = 2 * SW + 2 * TY;
 constexpr int NTHR = TX * TY;
 static_assert(NRING <= NTHR && NSLOT <= 2 * NTHR, "one ring slot and at most two footprint slots per thread");
+// phi footprint (two-cell halo) of the gradient stencils
+constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
+static_assert(NPSLOT <= 2 * NTHR, "at most two phi footprint slots per thread");
 
 struct Params {
   int nx, ny, nz;
@@ -111,6 +116,7 @@ struct Params {
   int gwall;                // 1: walls absorb phase populations (jet boundary)
   float nua, nub;           // colour model: viscosities of components A and B
   float beta;               // colour model: recolouring parameter
+  float tau;                // relaxation time (learned closure: omega = 1/(tau + 3 nu_t))
   float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
   float isc[TSLB_NFIELDS];  // 1 / sc
 };
@@ -359,15 +365,34 @@ __device__ __forceinline__ float grad_axis(const Get &get) {
   return 3.f * (wgt(1) * (fp - fm) + wgt(7) * (ep - em) + wgt(19) * (cp - cmn));
 }
 
+// unit normal n = grad(phi)/(|grad(phi)| + 1e-9) (recursive_2c_TSLB3D.f90:
+// 555-571); get(dx, dy, dz) returns phi of the neighbour
+template <typename Get>
+__device__ __forceinline__ void phi_normal(const Get &get, float &n0, float &n1, float &n2) {
+  const float gx = grad_axis<0>(get), gy = grad_axis<1>(get), gz = grad_axis<2>(get);
+  const float inv = 1.f / (sqrtf(gx * gx + gy * gy + gz * gz) + 1e-9f);
+  n0 = gx * inv;
+  n1 = gy * inv;
+  n2 = gz * inv;
+}
+
+// phi stencil coordinate along one axis: wrapped across a periodic face,
+// the nearest interior node across a wall
+__device__ __forceinline__ int clampw(int g, int n, int per) {
+  if (g >= 0 && g < n) return g;
+  return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
+}
+
 // ---------------------------------------------------------------------------
 // Source evaluation for k_fused: loads the moments of one node and returns its
 // shifted post-collision populations (and phase populations).
 // ---------------------------------------------------------------------------
-template <int NC>
+template <int NC, bool CLO = false>
 struct Src {
   float m[NC >= 2 ? 11 : 10];  // drho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz(,phi or psi)
   float nn[3];                 // normals (phase field) or colour gradient (colour model)
   float kap;                   // gated curvature (phase field)
+  float clo[CLO ? 7 : 1];      // learned closure: nu_t, dsigma (xx,yy,zz,xy,xz,yz)
 };
 
 // Raw buffer access.  A descriptor (gfx9 layout: 32-bit data format, no
@@ -423,6 +448,7 @@ __device__ __forceinline__ void st_off(T *__restrict__ base, uint32_t byte_off, 
 struct PlaneSrc {
   const char *f;      // state plane, field 0
   const char *n;      // normals plane, n0
+  const char *c;      // closure plane, nu_t
   uint32_t pbS, pb4;  // plane bytes (storage type, float)
   long long fsS, fs4; // field stride in bytes (storage type, float)
 };
@@ -431,21 +457,47 @@ struct PlaneSrc {
 // kOOB where the tile-plane has no interface (the buffer then returns 0
 // without a memory access).
 constexpr uint32_t kOOB = 0x80000000u;
-template <int NC, typename S>
-__device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint32_t off4,
-                                         const Params &p, Src<NC> &s) {
+template <int NC, typename S, bool CLO>
+__device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint32_t off4, uint32_t offC,
+                                         const Params &p, Src<NC, CLO> &s) {
 #pragma unroll
-  for (int q = 0; q < (NC >= 2 ? 11 : 10); q++) {
+  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) {  // phase field: phi from the LDS ring
     const float v = to_f(bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS));
     if constexpr (sizeof(S) == 4) s.m[q] = v;
     else s.m[q] = v * p.isc[q];
   }
-  if constexpr (NC >= 2) {
+  if constexpr (NC == 3) {  // colour gradient
     s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
     s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
     s.nn[2] = bld<float>(rsrc(ps.n + 2 * ps.fs4, ps.pb4), off4);
   }
+  // phase field: the gated curvature (the normals are computed by the caller)
   if constexpr (NC == 2) s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
+  if constexpr (CLO) {
+#pragma unroll
+    for (int q = 0; q < 7; q++) s.clo[q] = bld<float>(rsrc(ps.c + q * ps.fs4, ps.pb4), offC);
+  }
+}
+
+// Learned closure in the collision: omega = 1/(tau + 3 nu_t) and the stress
+// correction dsigma added to the post-collision non-equilibrium stress, so
+// that the populations are built from Pi* = (1 - omega) Pi + dsigma (and its
+// third-order terms).  Returns the moments to build from and their 1 - omega.
+template <int NC, bool CLO>
+__device__ __forceinline__ const float *closure_moments(const Src<NC, CLO> &s, const Params &p,
+                                                        float (&buf)[10], float &om1) {
+  if constexpr (!CLO) {
+    om1 = p.om1;
+    return s.m;
+  } else {
+    const float o1 = 1.f - 1.f / fmaf(3.f, s.clo[0], p.tau);
+#pragma unroll
+    for (int q = 0; q < 4; q++) buf[q] = s.m[q];
+#pragma unroll
+    for (int q = 4; q < 10; q++) buf[q] = fmaf(o1, s.m[q], s.clo[q - 3]);
+    om1 = 1.f;
+    return buf;
+  }
 }
 
 // Colour-gradient perturbation added to each population on its way to the
@@ -482,9 +534,11 @@ __device__ __forceinline__ float colour_a(float h, float ca, float bc, float gx,
   return fmaf(ca, h + wgt(L), bc * (wgt(L) * inv) * cg);
 }
 
-template <int NC, bool ADD, typename Sink>
-__device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, Sink &h, float *gq) {
+template <int NC, bool ADD, bool CLO, typename Sink>
+__device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p, Sink &h, float *gq) {
   const float *m = s.m;
+  float cbuf[10], om1;
+  const float *mc = closure_moments(s, p, cbuf, om1);
   if constexpr (NC == 2) {
     const float phi = m[10];
     const float n0 = s.nn[0], n1 = s.nn[1], n2 = s.nn[2];
@@ -495,18 +549,25 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, Sink
     const float esx = sx + p.fx + p.lin_a * m[1];
     const float esy = sy + p.fy + p.lin_a * m[2];
     const float esz = sz + p.fz + p.lin_a * m[3];
-    all_pops_mt<false>(m, m[1] + esx, m[2] + esy, m[3] + esz, p, h, p.om1);
+    all_pops_mt<false>(mc, m[1] + esx, m[2] + esy, m[3] + esz, p, h, om1);
     // D3Q7 phase field with sharpening, recursive_2c_TSLB3D.f90:808-831
+    // g_{+-a} = phi/8 +- A_a with A_a = (3/8) phi u_a + sh n_a: the sink gets
+    // A_x and A_y (the destination adds phi/8 of the source from the phi
+    // footprint), gq returns g0, A_x, A_y, phi/8, g5, g6
     const float p1g = 0.125f;
     const float sh = p1g * p.sharp_c * phi * (1.f - phi);
-    gq[0] = 0.25f * phi;
     const float base = p1g * phi;
-    h.template putg<1>(base * (1.f + 3.f * m[1]) + sh * n0);
-    h.template putg<2>(base * (1.f - 3.f * m[1]) - sh * n0);
-    h.template putg<3>(base * (1.f + 3.f * m[2]) + sh * n1);
-    h.template putg<4>(base * (1.f - 3.f * m[2]) - sh * n1);
-    gq[5] = base * (1.f + 3.f * m[3]) + sh * n2;
-    gq[6] = base * (1.f - 3.f * m[3]) - sh * n2;
+    const float ax = fmaf(3.f * base, m[1], sh * n0);
+    const float ay = fmaf(3.f * base, m[2], sh * n1);
+    const float az = fmaf(3.f * base, m[3], sh * n2);
+    h.template putg<1>(ax);
+    h.template putg<2>(ay);
+    gq[0] = 0.25f * phi;
+    gq[1] = ax;
+    gq[2] = ay;
+    gq[3] = base;
+    gq[5] = base + az;
+    gq[6] = base - az;
   } else if constexpr (NC == 3) {
     // colour model (recursive_colorTSLB3D.f90:719-959, corrected): per-node
     // omega from the harmonic-mean viscosity of the two components (:756-758)
@@ -529,10 +590,10 @@ __device__ __forceinline__ void eval_src(const Src<NC> &s, const Params &p, Sink
     gq[4] = gz;
   } else {
     if (p.shift) {
-      all_pops_mt<false>(m, m[1] + (p.fx + p.lin_a * m[1]), m[2] + (p.fy + p.lin_a * m[2]),
-                         m[3] + (p.fz + p.lin_a * m[3]), p, h, p.om1);
+      all_pops_mt<false>(mc, m[1] + (p.fx + p.lin_a * m[1]), m[2] + (p.fy + p.lin_a * m[2]),
+                         m[3] + (p.fz + p.lin_a * m[3]), p, h, om1);
     } else {
-      all_pops_mt<ADD>(m, m[1], m[2], m[3], p, h, p.om1);
+      all_pops_mt<ADD>(mc, m[1], m[2], m[3], p, h, om1);
     }
   }
 }
@@ -664,13 +725,17 @@ __device__ __forceinline__ float bb_sum_layer(const float (*lds)[NSLOT], int tx,
   return s;
 }
 
-template <int NC, bool MASK, bool ADD, int WPE, typename S>
+template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
     const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
-    const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, Params p) {
-  constexpr int NQ = NC == 2 ? 28 : 24;
+    const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, const float *__restrict__ clo,
+    Params p) {
+  constexpr int NQ = NC == 2 ? 26 : 24;
   constexpr int NF = NC >= 2 ? 11 : 10;
   __shared__ float lds[NQ][NSLOT];
+  // phase field: phi planes zs-1, zs, zs+1 of the (TX+4)x(TY+4) footprint
+  constexpr int NPR = NC == 2 ? 3 : 1;
+  __shared__ float phr[NPR][NC == 2 ? NPSLOT : 1];
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -709,6 +774,40 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   const int o_tile = (oy / TY) * p.tiles_x + ox / TX;
   const int r_tile = (ry / TY) * p.tiles_x + rx / TX;
 
+  // phase field: phi footprint slots tid and tid + NTHR of this thread, and
+  // their in-plane byte offsets (nearest interior node across a wall)
+  const bool ptwo = NC == 2 && tid + NTHR < NPSLOT;
+  auto phi_off = [&](int sl) -> uint32_t {
+    const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
+    const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
+    return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
+  };
+  const uint32_t pofs0 = NC == 2 ? phi_off(tid) : 0u, pofs1 = ptwo ? phi_off(tid + NTHR) : 0u;
+  S pr0{}, pr1{};  // raw storage values, converted at commit
+  auto phi_issue = [&](int z) {
+    const rsrc_t r = rsrc((const char *)src + TSLB_PHI * fsS + (size_t)clampw(z, p.nz, p.pz) * pbS, pbS);
+    pr0 = bld<S>(r, pofs0);
+    if (ptwo) pr1 = bld<S>(r, pofs1);
+  };
+  auto phi_commit = [&](float *d) {
+    if constexpr (sizeof(S) == 4) {
+      d[tid] = to_f(pr0);
+      if (ptwo) d[tid + NTHR] = to_f(pr1);
+    } else {
+      d[tid] = to_f(pr0) * p.isc[TSLB_PHI];
+      if (ptwo) d[tid + NTHR] = to_f(pr1) * p.isc[TSLB_PHI];
+    }
+  };
+  // normal of the source at footprint centre c and its phi (planes pm, p0, pp)
+  auto src_normal = [&](Src<NC, CLO> &sc, int c, const float *pm, const float *p0, const float *pp) {
+    auto get = [&](int dx, int dy, int dz) -> float {
+      const float *pl = dz < 0 ? pm : (dz > 0 ? pp : p0);
+      return pl[c + dy * PW + dx];
+    };
+    phi_normal(get, sc.nn[0], sc.nn[1], sc.nn[2]);
+    sc.m[NF - 1] = p0[c];
+  };
+
   // ghost-wall bounce-back bits of this column
   uint32_t wall_xy = 0;
   if (!p.px) {
@@ -732,12 +831,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   };
 
   // source fetch of one plane: own column and (tid < NRING) the ring slot
-  auto fetch = [&](int zs, Src<NC> &o, bool &ook, Src<NC> &r, bool &rok) {
+  auto fetch = [&](int zs, Src<NC, CLO> &o, bool &ook, Src<NC, CLO> &r, bool &rok) {
     int gz = 0;
     const bool zok = src_coord(zs, p.nz, p.pz, gz);
     PlaneSrc ps;
     ps.f = (const char *)src + (size_t)gz * pbS;
     ps.n = (const char *)nrm + (size_t)gz * pb4;
+    ps.c = (const char *)clo + (size_t)gz * pb4;
     ps.pbS = pbS;
     ps.pb4 = pb4;
     ps.fsS = fsS;
@@ -747,32 +847,39 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     ook = zok && oxy;
     if (ook) {
       if constexpr (MASK) ook = (bld_u32(lk, o_xy * 4u) & kSolid) == 0u;
-      bool un = NC != 2;  // flags gate the phase-field normals only
+      bool un = NC != 2;  // flags gate the phase-field curvature only
       if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)o_tile) != 0;
-      if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), un ? o_xy * 4u : kOOB, p, o);
+      if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), un ? o_xy * 4u : kOOB, o_xy * 4u, p, o);
     }
     rok = zok && rxy;
     if (rok) {
       if constexpr (MASK) rok = (bld_u32(lk, r_xy * 4u) & kSolid) == 0u;
-      bool un = NC != 2;  // flags gate the phase-field normals only
+      bool un = NC != 2;  // flags gate the phase-field curvature only
       if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)r_tile) != 0;
-      if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), un ? r_xy * 4u : kOOB, p, r);
+      if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), un ? r_xy * 4u : kOOB, r_xy * 4u, p, r);
     }
   };
 
   // One source plane zs.  aA, aB and aC accumulate the destination planes
   // zs-1 (completed and written here), zs and zs+1; bmA..bmC are their
   // bounce-back masks.  The loop below renames the three accumulators instead
-  // of copying them.
-  auto body = [&](int zs, Acc &aA, Acc &aB, Acc &aC, uint32_t &bmA, uint32_t &bmB, uint32_t &bmC) {
-    Src<NC> co, cr;
+  // of copying them.  Phase field: pm, p0 and pp hold phi of planes zs-1, zs
+  // and zs+1; plane zs+2 is loaded meanwhile and replaces zs-1 in pm.
+  auto body = [&](int zs, Acc &aA, Acc &aB, Acc &aC, uint32_t &bmA, uint32_t &bmB, uint32_t &bmC,
+                  float *pm, const float *p0, const float *pp) {
+    if constexpr (NC == 2) {
+      if (zs + 2 <= z1 + 1) phi_issue(zs + 2);  // planes up to z1 + 1 are read
+    }
+    Src<NC, CLO> co, cr;
     bool co_ok, cr_ok;
     fetch(zs, co, co_ok, cr, cr_ok);
     float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
+    float gax = 0.f, gay = 0.f, gb = 0.f;  // phase field: own A_x, A_y, phi/8
     float oc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // colour: cA, bc, g
     // ---- own source slot
     if (co_ok) {
+      if constexpr (NC == 2) src_normal(co, (ty + 2) * PW + tx + 2, pm, p0, pp);
       PopSink sk{lds, (ty + 1) * SW + (tx + 1), 0.f, 0.f, 0.f};
       eval_src<NC, ADD>(co, p, sk, gq);
       h0 = sk.h0;
@@ -780,6 +887,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       h6 = sk.h6;
       if constexpr (NC == 2) {
         g0 = gq[0];
+        gax = gq[1];
+        gay = gq[2];
+        gb = gq[3];
         g5 = gq[5];
         g6 = gq[6];
       }
@@ -788,12 +898,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     }
     // ---- halo ring slot
     if (cr_ok) {
+      if constexpr (NC == 2) src_normal(cr, (rsy + 1) * PW + rsx + 1, pm, p0, pp);
       PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
       eval_src<NC, ADD>(cr, p, sk, gq);
       if constexpr (NC == 3)
         for (int k = 0; k < 5; k++) rc[k] = gq[k];
     }
     __syncthreads();
+    if constexpr (NC == 2) phi_commit(pm);  // read again after the closing barrier
 
     bmC = plane_mask(zs + 1);
     acc_zero(aC);
@@ -803,9 +915,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         gather_layer<0, false>(aB, lds, tx, ty, h0, 0u);
         gather_layer<1, false>(aC, lds, tx, ty, h5, 0u);
         if constexpr (NC == 2) {
+          // g_{+-a} of the in-plane neighbours: phi/8 from the phi plane, +-A_a from LDS
           aA.phi += g6;
-          aB.phi += g0 + lds[24][(ty + 1) * SW + tx] + lds[25][(ty + 1) * SW + tx + 2] +
-                    lds[26][ty * SW + tx + 1] + lds[27][(ty + 2) * SW + tx + 1];
+          aB.phi += g0 + fmaf(0.125f, p0[(ty + 2) * PW + tx + 1], lds[24][(ty + 1) * SW + tx]) +
+                    fmaf(0.125f, p0[(ty + 2) * PW + tx + 3], -lds[24][(ty + 1) * SW + tx + 2]) +
+                    fmaf(0.125f, p0[(ty + 1) * PW + tx + 2], lds[25][ty * SW + tx + 1]) +
+                    fmaf(0.125f, p0[(ty + 3) * PW + tx + 2], -lds[25][(ty + 2) * SW + tx + 1]);
           aC.phi += g5;
         }
       } else {
@@ -816,15 +931,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         bb_layer<0>(aB, lds, tx, ty, h0, h5, h6, bmB);
         bb_layer<1>(aB, lds, tx, ty, h0, h5, h6, bmB);
         if constexpr (NC == 2) {
-          const int own = (ty + 1) * SW + tx + 1;
-          const float gown[7] = {g0, lds[24][own], lds[25][own], lds[26][own], lds[27][own], g5, g6};
+          const float gown[7] = {g0, gb + gax, gb - gax, gb + gay, gb - gay, g5, g6};
           // phase populations: l=6 into plane zs-1, l=0..4 into zs, l=5 into zs+1
           if (!((bmA >> 6) & 1u)) aA.phi += g6;
           float s = g0;
-          if (!((bmB >> 1) & 1u)) s += lds[24][(ty + 1) * SW + tx];
-          if (!((bmB >> 2) & 1u)) s += lds[25][(ty + 1) * SW + tx + 2];
-          if (!((bmB >> 3) & 1u)) s += lds[26][ty * SW + tx + 1];
-          if (!((bmB >> 4) & 1u)) s += lds[27][(ty + 2) * SW + tx + 1];
+          if (!((bmB >> 1) & 1u)) s += fmaf(0.125f, p0[(ty + 2) * PW + tx + 1], lds[24][(ty + 1) * SW + tx]);
+          if (!((bmB >> 2) & 1u)) s += fmaf(0.125f, p0[(ty + 2) * PW + tx + 3], -lds[24][(ty + 1) * SW + tx + 2]);
+          if (!((bmB >> 3) & 1u)) s += fmaf(0.125f, p0[(ty + 1) * PW + tx + 2], lds[25][ty * SW + tx + 1]);
+          if (!((bmB >> 4) & 1u)) s += fmaf(0.125f, p0[(ty + 3) * PW + tx + 2], -lds[25][(ty + 2) * SW + tx + 1]);
           if (!p.gwall) {
 #pragma unroll
             for (int l = 1; l <= 6; l++)
@@ -907,12 +1021,20 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   acc_zero(a0);
   acc_zero(a1);
   uint32_t m0 = 0, m1 = plane_mask(z0 - 1), m2 = 0;
+  float *R0 = phr[0], *R1 = phr[NPR > 1 ? 1 : 0], *R2 = phr[NPR > 2 ? 2 : 0];
+  if constexpr (NC == 2) {
+    for (int k = 0; k < 3; k++) {
+      phi_issue(z0 - 2 + k);
+      phi_commit(phr[k < NPR ? k : 0]);
+    }
+    __syncthreads();
+  }
   for (int zs = z0 - 1; zs <= z1; zs += 3) {
-    body(zs, a0, a1, a2, m0, m1, m2);
+    body(zs, a0, a1, a2, m0, m1, m2, R0, R1, R2);
     if (zs + 1 > z1) break;
-    body(zs + 1, a1, a2, a0, m1, m2, m0);
+    body(zs + 1, a1, a2, a0, m1, m2, m0, R1, R2, R0);
     if (zs + 2 > z1) break;
-    body(zs + 2, a2, a0, a1, m2, m0, m1);
+    body(zs + 2, a2, a0, a1, m2, m0, m1, R2, R0, R1);
   }
 }
 
@@ -926,17 +1048,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 // the current plane is computed.  RAW (colour model) writes the unnormalised
 // gradient of the colour field psi and no curvature.
 //
-// Sparse output: n and kappa are zero wherever phi is locally uniform (the
-// bulk of both phases), so each (tile, plane) gets a flag byte, 1 when any of
-// its n or kappa is nonzero; n and kappa are written only for flagged
-// tile-planes, and the fused pass reads them only there.  A plane whose phi
-// window z-1..z+1 is uniform skips the stencil; uniformity of each loaded
-// plane is voted per wave from registers.  `dense` writes everything.
-constexpr int PW = TX + 4, PH = TY + 4, NPSLOT = PW * PH;
-static_assert(NPSLOT <= 2 * NTHR, "at most two phi footprint slots per thread");
+// Sparse output (phase field): the fused pass computes n itself from phi and
+// needs only kappa, which is zero outside a thin shell round the interface;
+// each (tile, plane) gets a flag byte, 1 when any of its kappa is nonzero, and
+// kappa is written only for flagged tile-planes, where the fused pass reads
+// it.  A plane whose phi window z-1..z+1 is uniform skips the stencil;
+// uniformity of each loaded plane is voted per wave from registers.  `dense`
+// writes n and kappa everywhere (and flags every tile-plane); RAW is always
+// dense.
 constexpr int NWAVE = NTHR / 64;
 template <typename S, bool RAW>
-__global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ phi,
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_normals_tiled(const S *__restrict__ phi,
                                                          float *__restrict__ n0,
                                                          float *__restrict__ n1,
                                                          float *__restrict__ n2,
@@ -946,7 +1068,7 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
                                                          int has_mask, int dense, Params p) {
   __shared__ float ph[4][NPSLOT];
   __shared__ float nl[4][3][NSLOT];
-  __shared__ float wv[NWAVE][2];  // per wave: plane uniform (1/0), its value
+  __shared__ float wv[NWAVE][3];  // per wave: plane uniform (1/0), its value, interface (1/0)
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -961,10 +1083,6 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
   const bool active = x < p.nx && y < p.ny;
   const size_t plane = (size_t)p.nx * p.ny;
   const size_t tiles_plane = (size_t)p.tiles_x * p.tiles_y;
-  auto clampw = [](int g, int n, int per) -> int {
-    if (g >= 0 && g < n) return g;
-    return per ? wrapc(g, n) : (g < 0 ? 0 : n - 1);
-  };
   // ring slot of plane z >= z0 - 2
   auto r4 = [z0](int z) { return (z - z0 + 4) & 3; };
 
@@ -976,36 +1094,64 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
     return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
   };
   const uint32_t pofs0 = phi_off(tid), pofs1 = two ? phi_off(tid + NTHR) : 0u;
-  float pr0 = 0.f, pr1 = 0.f;
+  // is a phi slot one of this tile's own nodes?
+  auto phi_own = [&](int sl) -> bool {
+    const int sx = sl % PW - 2, sy = sl / PW - 2;
+    return sx >= 0 && sx < TX && sy >= 0 && sy < TY && x0 + sx < p.nx && y0 + sy < p.ny;
+  };
+  const bool pown0 = phi_own(tid), pown1 = two && phi_own(tid + NTHR);
   const uint32_t pbS = (uint32_t)(plane * sizeof(S)), pb4 = (uint32_t)(plane * 4);
-  auto issue = [&](int z) {
+  // phi planes are loaded PQ planes ahead of their commit into a register
+  // queue of raw storage values (converted at commit, so that no instruction
+  // waits on a load before its plane is due)
+  struct PhiQ {
+    S a, b;
+  };
+  auto issue = [&](int z, PhiQ &q) {
+    if (z > z1 + 1) return;  // planes up to z1 + 1 are read
     const rsrc_t base = rsrc(phi + plane * (size_t)clampw(z, p.nz, p.pz), pbS);
-    pr0 = to_f(bld<S>(base, pofs0));
-    if (two) pr1 = to_f(bld<S>(base, pofs1));
+    q.a = bld<S>(base, pofs0);
+    if (two) q.b = bld<S>(base, pofs1);
+  };
+  float pr0 = 0.f, pr1 = 0.f;
+  auto take = [&](const PhiQ &q) {
+    pr0 = to_f(q.a);
+    pr1 = two ? to_f(q.b) : 0.f;
     if constexpr (sizeof(S) != 4) {
       pr0 *= p.isc[TSLB_PHI];
       pr1 *= p.isc[TSLB_PHI];
     }
   };
-  // store the loaded plane z and post this wave's vote: all its values equal?
+  // store the loaded plane z and post this wave's votes: are all its values
+  // equal, and may one of the tile's own nodes carry curvature?  The gate
+  // phi(1-phi)|n|^2 > 0.2 with |n| < 1 needs phi(1-phi) > 0.2; 0.19 leaves
+  // room for rounding.
   auto commit = [&](int z) {
     float *d = ph[r4(z)];
     d[tid] = pr0;
     if (two) d[tid + NTHR] = pr1;
     const float v = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pr0)));
     const bool all = __all(pr0 == v && (!two || pr1 == v));
+    const bool hit = (pown0 && pr0 * (1.f - pr0) > 0.19f) || (pown1 && pr1 * (1.f - pr1) > 0.19f);
+    const bool ihit = __any(hit);
     if ((tid & 63) == 0) {
       wv[tid >> 6][0] = all ? 1.f : 0.f;
       wv[tid >> 6][1] = v;
+      wv[tid >> 6][2] = ihit ? 1.f : 0.f;
     }
   };
-  // after a barrier: is the plane of the last commit one value (and which)?
-  auto read_vote = [&](float &val) -> bool {
+  // after a barrier: is the plane of the last commit one value (and which),
+  // and does it hold interface nodes of this tile (iface)?
+  auto read_vote = [&](float &val, bool &iface) -> bool {
     const float v0 = wv[0][1];
-    bool u = true;
+    bool u = true, i = false;
 #pragma unroll
-    for (int w = 0; w < NWAVE; w++) u = u && wv[w][0] != 0.f && wv[w][1] == v0;
+    for (int w = 0; w < NWAVE; w++) {
+      u = u && wv[w][0] != 0.f && wv[w][1] == v0;
+      i = i || wv[w][2] != 0.f;
+    }
     val = v0;
+    iface = i;
     return u;
   };
 
@@ -1041,16 +1187,12 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
             const float *pl = dz < 0 ? P0 : (dz > 0 ? P2 : P1);
             return pl[c + dy * PW + dx];
           };
-          const float gx_ = grad_axis<0>(get), gy_ = grad_axis<1>(get), gz_ = grad_axis<2>(get);
           if constexpr (RAW) {
-            a = gx_;
-            bb = gy_;
-            cc = gz_;
+            a = grad_axis<0>(get);
+            bb = grad_axis<1>(get);
+            cc = grad_axis<2>(get);
           } else {
-            const float inv = 1.f / (sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_) + 1e-9f);
-            a = gx_ * inv;
-            bb = gy_ * inv;
-            cc = gz_ * inv;
+            phi_normal(get, a, bb, cc);
           }
         }
       }
@@ -1062,17 +1204,29 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
     if (ntwo) one(tid + NTHR, nc1, nxy1, nok1);
   };
 
-  // uniformity (u*) and value (v*) of phi planes z-2 .. z+2 for the current z
+  // uniformity (u*), value (v*) and interface votes (i*) of phi planes
+  // z-2 .. z+2 for the current z.  Sparse phase-field output (gate) needs the
+  // curvature of plane z only where i(z), and so n of plane z only where one
+  // of i(z-1), i(z), i(z+1) holds; elsewhere both stay zero.
+  const bool gate = !RAW && !dense;
   bool um2, um1, u0, up1, up2;
   float vm2, vm1, v0, vp1, vp2;
-  bool us[4];
+  bool i0, ip1, ip2;
+  bool us[4], is[4];
   float vs[4];
+  PhiQ q0, q1, q2, q3;  // plane z lives in q[(z - z0 + 2) & 3]
+  issue(z0 - 2, q0);
+  issue(z0 - 1, q1);
+  issue(z0, q2);
+  issue(z0 + 1, q3);
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    issue(z0 - 2 + k);
+    PhiQ &q = k == 0 ? q0 : (k == 1 ? q1 : (k == 2 ? q2 : q3));
+    take(q);
     commit(z0 - 2 + k);
+    issue(z0 + 2 + k, q);
     __syncthreads();
-    us[k] = read_vote(vs[k]);
+    us[k] = read_vote(vs[k], is[k]);
     __syncthreads();
   }
   um2 = us[0];
@@ -1083,18 +1237,21 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
   v0 = vs[2];
   up1 = us[3];
   vp1 = vs[3];
-  normals_plane(z0 - 1, um2 && um1 && u0 && vm2 == vm1 && vm1 == v0);
-  normals_plane(z0, um1 && u0 && up1 && vm1 == v0 && v0 == vp1);
-  issue(z0 + 2);
+  i0 = is[2];
+  ip1 = is[3];
+  normals_plane(z0 - 1, (um2 && um1 && u0 && vm2 == vm1 && vm1 == v0) || (gate && !i0));
+  normals_plane(z0, (um1 && u0 && up1 && vm1 == v0 && v0 == vp1) || (gate && !(i0 || ip1)));
   __syncthreads();
-  for (int z = z0; z < z1; z++) {
+  // one destination plane z; q holds plane z + 2 and is refilled with z + 6
+  auto body = [&](int z, PhiQ &q) {
+    take(q);
     commit(z + 2);
-    if (z + 3 <= z1 + 1) issue(z + 3);  // planes up to z1 + 1 are read
+    issue(z + 6, q);
     __syncthreads();
-    up2 = read_vote(vp2);
-    normals_plane(z + 1, u0 && up1 && up2 && v0 == vp1 && vp1 == vp2);
+    up2 = read_vote(vp2, ip2);
+    normals_plane(z + 1, (u0 && up1 && up2 && v0 == vp1 && vp1 == vp2) || (gate && !(i0 || ip1 || ip2)));
     __syncthreads();
-    const bool zero_z = um1 && u0 && up1 && vm1 == v0 && v0 == vp1;
+    const bool zero_z = (um1 && u0 && up1 && vm1 == v0 && v0 == vp1) || (gate && !i0);
     int any = 0;
     float a = 0.f, bb = 0.f, c = 0.f, curv = 0.f;
     if (!zero_z && active) {
@@ -1117,17 +1274,20 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
         const float dzz = grad_axis<2>(nat(2));
         curv = -(dxx + dyy + dzz);
       }
-      any = (a != 0.f) | (bb != 0.f) | (c != 0.f) | (curv != 0.f);
+      if constexpr (RAW) any = (a != 0.f) | (bb != 0.f) | (c != 0.f);
+      else any = curv != 0.f;
     }
     // block-uniform: skip the vote for a uniform window
     const int flag = zero_z ? 0 : __syncthreads_or(any);
     if ((flag || dense) && active) {
       const size_t zb = plane * (size_t)z;
       const uint32_t off = (uint32_t)(x + p.nx * y) * 4u;
-      bst(rsrc(n0 + zb, pb4), off, a);
-      bst(rsrc(n1 + zb, pb4), off, bb);
-      bst(rsrc(n2 + zb, pb4), off, c);
-      bst(rsrc(kap + zb, pb4), off, curv);
+      if (RAW || dense) {
+        bst(rsrc(n0 + zb, pb4), off, a);
+        bst(rsrc(n1 + zb, pb4), off, bb);
+        bst(rsrc(n2 + zb, pb4), off, c);
+      }
+      if constexpr (!RAW) bst(rsrc(kap + zb, pb4), off, curv);
     }
     if (tid == 0) flags[tiles_plane * z + tile_y * p.tiles_x + tile_x] = (uint8_t)(flag || dense);
     um2 = um1;
@@ -1138,6 +1298,17 @@ __global__ __launch_bounds__(NTHR) void k_normals_tiled(const S *__restrict__ ph
     v0 = vp1;
     up1 = up2;
     vp1 = vp2;
+    i0 = ip1;
+    ip1 = ip2;
+  };
+  for (int z = z0; z < z1; z += 4) {
+    body(z, q0);
+    if (z + 1 >= z1) break;
+    body(z + 1, q1);
+    if (z + 2 >= z1) break;
+    body(z + 2, q2);
+    if (z + 3 >= z1) break;
+    body(z + 3, q3);
   }
 }
 
@@ -1302,11 +1473,14 @@ __global__ __launch_bounds__(256) void k_jet_bc(const S *__restrict__ src, S *__
 #pragma unroll
   for (int q = 0; q < (NC == 2 ? 11 : 10); q++) sy.m[q] = at(q, y);
   if constexpr (NC == 2) {
+    // n from phi as the fused pass computes it; planes zy +- 1 are in range
+    auto get = [&](int dx, int dy, int dz) -> float {
+      const int gi = clampw(i + dx, p.nx, p.px), gj = clampw(j + dy, p.ny, p.py);
+      return at(TSLB_PHI, gi + p.nx * gj + plane * (zy + dz));
+    };
+    phi_normal(get, sy.nn[0], sy.nn[1], sy.nn[2]);
     const int tile = (j / TY) * p.tiles_x + i / TX;
     const bool un = flags[(size_t)zy * p.tiles_x * p.tiles_y + tile] != 0;
-    sy.nn[0] = un ? nrm[y] : 0.f;
-    sy.nn[1] = un ? nrm[(size_t)p.fs + y] : 0.f;
-    sy.nn[2] = un ? nrm[2 * (size_t)p.fs + y] : 0.f;
     sy.kap = un ? nrm[3 * (size_t)p.fs + y] : 0.f;
   }
   ArraySink own;
@@ -1445,6 +1619,8 @@ __global__ __launch_bounds__(256) void k_fill(float *__restrict__ a, float v, in
   if (id < n) a[id] = v;
 }
 
+#include "closure_kernels.h"
+
 // ---------------------------------------------------------------------------
 // FP64 diagnostics: per-block partial sums, then one block sums the partials.
 // ---------------------------------------------------------------------------
@@ -1550,6 +1726,12 @@ struct tslb_ctx {
   int jet = 0;     // turbulent-jet inlet/outlet planes (k_jet_bc after k_fused)
   int zwalls = 0;  // bit 0: plane 0 all solid; bit 1: plane nz-1 all solid
   int variant = 2;  // bit 1: cap VGPRs at 128 (two blocks per CU)
+  // learned closure: 0 none, 1 fixed per-node fields, 2 recomputed before
+  // every collision by `model` from the current state
+  int clo_mode = 0;
+  float *clo = nullptr;    // nu_t, dsigma[6]: 7 * fs floats
+  float *cfeat = nullptr;  // closure inputs of every node: 10 * n floats
+  struct tslb_closure *model = nullptr;
   size_t bytes = 0;
   int timing = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -1633,6 +1815,8 @@ void zero_fields(tslb_ctx *c, void *base, int q0, int q1) {
                      (int)words);
 }
 
+int launch_closure(tslb_ctx *c);  // closure_api.h
+
 int launch_normals(tslb_ctx *c, int dense = 0) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
@@ -1680,8 +1864,16 @@ int launch_fused(tslb_ctx *c) {
     const S *s = (const S *)c->buf[c->cur];
     S *d = (S *)c->buf[1 - c->cur];
 #define TSLB_LAUNCH(NC, MASK, ADD, WPE)                                                          \
-  hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, d, \
-                     c->nrm, c->flags, c->links, p)
+  do {                                                                                             \
+    if (c->clo_mode) {                                                                             \
+      if constexpr (NC != TSLB_COLOUR)                                                             \
+        hipLaunchKernelGGL((k_fused<NC, MASK, ADD, 4, S, true>), dim3(nblk), dim3(NTHR), 0,        \
+                           c->stream, s, d, c->nrm, c->flags, c->links, c->clo, p);                \
+    } else {                                                                                       \
+      hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, \
+                         d, c->nrm, c->flags, c->links, (const float *)nullptr, p);                \
+    }                                                                                              \
+  } while (0)
     // FP32 storage keeps both register budgets selectable for sweeps (capped
     // at 128 VGPRs for two blocks per CU, or uncapped); reduced storage is capped
 #define TSLB_LAUNCH_V(NC, MASK, ADD)                                      \
@@ -1834,6 +2026,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.nub = ((p->tau_b > 0.5f ? p->tau_b : p->tau) - 0.5f) / 3.f;
   k.beta = p->beta;
   k.om1 = 1.f - 1.f / p->tau;
+  k.tau = p->tau;
   k.sigma = p->sigma;
   k.sharp_c = p->sharp_c;
   k.fx = p->force[0];
@@ -1924,6 +2117,8 @@ int tslb_destroy(tslb_ctx *c) {
   (void)hipFree(c->red);
   (void)hipFree(c->w_in);
   (void)hipFree(c->phi_in);
+  (void)hipFree(c->clo);
+  (void)hipFree(c->cfeat);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TSLB_OK;
@@ -2157,6 +2352,7 @@ int tslb_collide_stream(tslb_ctx *c) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (c->prm.ncomp >= 2 && !c->normals_fresh) return fail(TSLB_E_STATE, "tslb_normals() must precede tslb_collide_stream()");
+  if (c->clo_mode == 2 && (rc = launch_closure(c))) return rc;
   return launch_fused(c);
 }
 
@@ -2165,6 +2361,7 @@ int tslb_step(tslb_ctx *c, int nsteps) {
   if (rc) return rc;
   if (nsteps < 0) return fail(TSLB_E_ARG, "nsteps < 0");
   for (int s = 0; s < nsteps; s++) {
+    if (c->clo_mode == 2 && (rc = launch_closure(c))) return rc;
     if (c->prm.ncomp >= 2 && (rc = launch_normals(c))) return rc;
     if ((rc = launch_fused(c))) return rc;
     if ((rc = launch_zbc(c))) return rc;
@@ -2185,6 +2382,7 @@ int tslb_set_jet_bc(tslb_ctx *c, int enable) {
   if (!c->has_mask || c->zwalls != 3)
     return fail(TSLB_E_STATE, "jet boundary needs every node of planes k=1 and k=nz solid (tslb_set_isfluid)");
   if (c->zbc) return fail(TSLB_E_STATE, "jet boundary and tslb_set_zplane_bc are exclusive");
+  if (c->clo_mode) return fail(TSLB_E_STATE, "jet boundary and the learned closure are exclusive");
   if (c->prm.ncomp == TSLB_COLOUR) return fail(TSLB_E_STATE, "the jet boundary is defined for the phase-field model");
   c->jet = 1;
   c->kp.gwall = 1;
@@ -2365,3 +2563,5 @@ int tslb_last_error(char *buf, int len) {
 const char *tslb_version(void) { return "tslb 0.1 gfx950"; }
 
 }  // extern "C"
+
+#include "closure_api.h"

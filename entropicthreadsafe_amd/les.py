@@ -1,0 +1,219 @@
+"""Closure LES: the learned subgrid closure judged against the plain method.
+
+A fine two-component run (emulsion in linearly forced turbulence) supplies
+box-filtered snapshots for training the closure (tslb_closure_add_samples,
+tslb_closure_train, all on the GPU).  From the filtered fine state at time
+t0, two coarse runs follow the flow for the same physical time (acoustic
+scaling: dx and dt both grow by the filter factor r): the plain coarse
+lattice and the same lattice with the trained closure attached.  Each is
+compared with the filtered fine run at the same time through the shell-
+averaged kinetic-energy spectrum E(k) over the resolved band, and the wall
+time of the closure run is compared with the fine run's.
+
+Parameter scaling to the coarse grid (lattice units): nu_c = nu_f / r,
+sigma_c = sigma_f / r, linear-forcing rate A_c = r A_f; phi and velocities
+carry over.  The initial velocity is a seeded solenoidal random field with
+spectrum E(k) ~ k^4 exp(-2 (k/k0)^2) (generated with torch FFTs on the
+device; plumbing, not the product path), plus the droplets of the emulsion
+workload.
+"""
+import json
+import time
+
+import numpy as np
+
+from . import configs
+from .tslb import Closure, Lattice
+
+
+def solenoidal_field(n, k0, urms, seed, device="cuda:0"):
+    """Random divergence-free velocity (3, n, n, n) float32 with spectrum
+    ~ k^4 exp(-2 (k/k0)^2), scaled to the given rms per component."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    k1 = torch.fft.fftfreq(n, d=1.0 / n, device=device)
+    kr = torch.fft.rfftfreq(n, d=1.0 / n, device=device)
+    kz, ky, kx = torch.meshgrid(k1, k1, kr, indexing="ij")
+    k2 = kx * kx + ky * ky + kz * kz
+    kk = torch.sqrt(k2)
+    amp = kk * kk * torch.exp(-(kk / k0) ** 2) / torch.clamp(kk, min=1.0)  # sqrt(E(k)/(4 pi k^2))
+    amp[0, 0, 0] = 0.0
+    out = []
+    comps = []
+    for _ in range(3):
+        re = torch.randn(kz.shape, generator=g, device=device)
+        im = torch.randn(kz.shape, generator=g, device=device)
+        comps.append(torch.complex(re, im) * amp)
+    # project out the compressive part: a - k (k.a)/k^2
+    dot = (kx * comps[0] + ky * comps[1] + kz * comps[2]) / torch.clamp(k2, min=1.0)
+    for c, kc in zip(comps, (kx, ky, kz)):
+        a = c - kc * dot
+        f = torch.fft.irfftn(a, s=(n, n, n))
+        out.append(f)
+    u = torch.stack(out)
+    u = u * (urms / torch.sqrt((u * u).mean()))
+    return u.float().cpu().numpy()
+
+
+def box(a, r):
+    """r^3 block means of a (nz, ny, nx)."""
+    nz, ny, nx = a.shape
+    return a.reshape(nz // r, r, ny // r, r, nx // r, r).mean(axis=(1, 3, 5), dtype=np.float64)
+
+
+def filtered_state(lat, r):
+    """rho_bar, Favre velocity and phi_bar of a lattice (host, float64)."""
+    rho = lat.download_field("rho").astype(np.float64)
+    rb = box(rho, r)
+    vel = [box(rho * lat.download_field(c), r) / rb for c in ("u", "v", "w")]
+    phi = box(lat.download_field("phi"), r)
+    return rb, vel, phi
+
+
+def energy_spectrum(u, v, w):
+    """Shell-averaged kinetic-energy spectrum E(k), k = 0 .. n/2."""
+    n = u.shape[0]
+    e = np.zeros(n // 2 + 1)
+    k1 = np.fft.fftfreq(n, 1.0 / n)
+    kr = np.fft.rfftfreq(n, 1.0 / n)
+    kz, ky, kx = np.meshgrid(k1, k1, kr, indexing="ij")
+    kk = np.rint(np.sqrt(kx * kx + ky * ky + kz * kz)).astype(int)
+    wgt = np.where(kx == 0, 1.0, 2.0)   # rfft half-space
+    if n % 2 == 0:
+        wgt = np.where(kx == n // 2, 1.0, wgt)
+    tot = np.zeros_like(kk, dtype=np.float64)
+    for a in (u, v, w):
+        f = np.fft.rfftn(a) / a.size
+        tot += 0.5 * wgt * (f.real ** 2 + f.imag ** 2)
+    m = kk <= n // 2
+    np.add.at(e, kk[m], tot[m])
+    return e
+
+
+def spectrum_error(e, ref, kmax):
+    """Mean relative deviation of E(k) from the reference over 1 <= k <= kmax."""
+    k = np.arange(1, kmax + 1)
+    return float(np.mean(np.abs(e[k] / ref[k] - 1.0)))
+
+
+def _coarse_lattice(nc, kw, r, rb, vel, phi):
+    ckw = dict(kw)
+    ckw["tau"] = 0.5 + (kw["tau"] - 0.5) / r
+    ckw["sigma"] = kw["sigma"] / r
+    ckw["lin_force"] = kw["lin_force"] * r
+    lat = Lattice(nc, nc, nc, **ckw)
+    lat.upload_field("rho", rb.astype(np.float32))
+    for c, a in zip(("u", "v", "w"), vel):
+        lat.upload_field(c, a.astype(np.float32))
+    lat.upload_field("phi", phi.astype(np.float32))
+    lat.init_equilibrium(True)
+    return lat
+
+
+def _timed_steps(lat, steps):
+    lat.sync()
+    t = time.perf_counter()
+    lat.step(steps)
+    lat.sync()
+    return time.perf_counter() - t
+
+
+def run(n=512, r=4, k0=4.0, urms=0.04, spinup=1500, snaps=4, snap_every=250, horizon=2000,
+        train_steps=3000, batch=16384, lr=2e-3, seed=1, log=print):
+    """The experiment; returns a dict of its results."""
+    case = configs.emulsion(n=n)
+    kw = dict(case.lattice)
+    nc = n // r
+    res = dict(workload=f"closure-les emulsion-2c {n}^3 -> {nc}^3", n_fine=n, n_coarse=nc, factor=r,
+               spinup=spinup, horizon_fine_steps=horizon, horizon_coarse_steps=horizon // r)
+    vel0 = solenoidal_field(n, k0, urms, seed)
+    with Lattice(n, n, n, **kw) as fine, Closure(max_samples=snaps * nc ** 3) as model:
+        fine.upload_field("rho", case.fields["rho"])
+        for c, a in zip(("u", "v", "w"), vel0):
+            fine.upload_field(c, a)
+        fine.upload_field("phi", case.fields["phi"])
+        fine.init_equilibrium(True)
+        del vel0
+        t_spin = _timed_steps(fine, spinup)
+        log(f"les: fine spin-up {spinup} steps in {t_spin:.1f} s")
+        for s in range(snaps):
+            fine.step(snap_every)
+            model.add_samples(fine, r)
+        log(f"les: {model.num_samples} training samples from {snaps} snapshots")
+        rb, vel, phi = filtered_state(fine, r)     # coarse initial state, time t0
+        t_fine = _timed_steps(fine, horizon)
+        frb, fvel, _ = filtered_state(fine, r)
+        e_ref = energy_spectrum(*fvel)
+        res["fine_seconds"] = t_fine
+        log(f"les: fine horizon {horizon} steps in {t_fine:.2f} s")
+
+        # training: inputs scaled to unit spread, outputs to the target size
+        feat, aux = model.get_samples()
+        fstd = feat.std(axis=1) + 1e-12
+        t_scale = float(np.sqrt((aux[1:] ** 2).mean()))
+        theta0 = _init_theta(seed, 1.0 / fstd, t_scale)
+        model.set_params(theta0)
+        l0 = model.loss_grad(t_scale, grad=False)
+        t0 = time.perf_counter()
+        last = model.train(train_steps, batch, lr, t_scale, seed=seed)
+        t_train = time.perf_counter() - t0
+        l1 = model.loss_grad(t_scale, grad=False)
+        res.update(train_loss_initial=l0, train_loss_final=l1, train_seconds=t_train, train_steps=train_steps,
+                   batch=batch, t_scale=t_scale, samples=model.num_samples)
+        log(f"les: training loss {l0:.4g} -> {l1:.4g} ({train_steps} Adam steps in {t_train:.1f} s)")
+
+        # the coarse runs
+        spectra = {}
+        for name in ("plain", "closure"):
+            with _coarse_lattice(nc, kw, r, rb, vel, phi) as lat:
+                if name == "closure":
+                    lat.set_closure(model)
+                dt = _timed_steps(lat, horizon // r)
+                m = lat.download()
+                if name == "closure":
+                    lat.set_closure(None)
+            ok = bool(np.isfinite(m).all())
+            spectra[name] = energy_spectrum(m[1], m[2], m[3]) if ok else None
+            res[f"{name}_seconds"] = dt
+            res[f"{name}_finite"] = ok
+            log(f"les: coarse {name} {horizon // r} steps in {dt:.3f} s, finite={ok}")
+    kmax = nc // 4
+    res["kmax"] = kmax
+    for name, e in spectra.items():
+        res[f"{name}_spectrum_error"] = spectrum_error(e, e_ref, kmax) if e is not None else None
+    res["speedup_closure_vs_fine"] = res["fine_seconds"] / res["closure_seconds"]
+    res["spectrum_k"] = list(range(kmax + 1))
+    res["spectrum_fine_filtered"] = [float(x) for x in e_ref[:kmax + 1]]
+    for name, e in spectra.items():
+        res[f"spectrum_{name}"] = [float(x) for x in e[:kmax + 1]] if e is not None else None
+    return res
+
+
+def _init_theta(seed, in_scale, t_scale):
+    """Random MLP weights (He), zero biases; nu_t output scale 1e-4 and the
+    stress corrections scaled to the target size."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i, o in ((10, 64), (64, 64), (64, 64), (64, 7)):
+        parts += [(rng.standard_normal((i, o)) * np.sqrt(2.0 / i)).ravel(), np.zeros(o)]
+    parts[-2] = parts[-2] * 0.1            # small initial output layer
+    parts += [np.asarray(in_scale, np.float64), np.array([1e-4] + [t_scale] * 6)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--factor", type=int, default=4)
+    ap.add_argument("--spinup", type=int, default=1500)
+    ap.add_argument("--horizon", type=int, default=2000)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    out = run(n=a.n, r=a.factor, spinup=a.spinup, horizon=a.horizon, train_steps=a.train_steps)
+    js = json.dumps(out)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(js + "\n")
+    print(json.dumps({k: v for k, v in out.items() if not k.startswith("spectrum")}))

@@ -83,6 +83,23 @@ SIGNATURES = {
     "tslb_device_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "tslb_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "tslb_version": (ctypes.c_char_p, []),
+    "tslb_closure_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int64]),
+    "tslb_closure_destroy": (ctypes.c_int, [_P]),
+    "tslb_closure_set_params": (ctypes.c_int, [_P, _FP, ctypes.c_int]),
+    "tslb_closure_get_params": (ctypes.c_int, [_P, _FP, ctypes.c_int]),
+    "tslb_closure_add_samples": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+    "tslb_closure_num_samples": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "tslb_closure_clear_samples": (ctypes.c_int, [_P]),
+    "tslb_closure_get_samples": (ctypes.c_int, [_P, _FP, _FP]),
+    "tslb_closure_set_samples": (ctypes.c_int, [_P, _FP, _FP, ctypes.c_int64]),
+    "tslb_closure_forward": (ctypes.c_int, [_P, _FP]),
+    "tslb_closure_loss_grad": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64, ctypes.c_float,
+                                              _DP, _FP]),
+    "tslb_closure_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_uint32, _DP]),
+    "tslb_set_closure": (ctypes.c_int, [_P, _P]),
+    "tslb_set_closure_fields": (ctypes.c_int, [_P, _FP]),
+    "tslb_download_closure": (ctypes.c_int, [_P, _FP]),
 }
 
 _lib = None
@@ -311,3 +328,119 @@ class Lattice:
         b = ctypes.c_size_t()
         _check(self.lib.tslb_device_bytes(self._h, ctypes.byref(b)), "tslb_device_bytes")
         return b.value
+
+    # -- learned closure ------------------------------------------------------
+    def set_closure(self, model):
+        """Attach a Closure (recomputed before every collision); None detaches."""
+        _check(self.lib.tslb_set_closure(self._h, model._h if model is not None else None), "tslb_set_closure")
+        self._closure = model
+
+    def set_closure_fields(self, clo):
+        """Fixed per-node closure fields clo (7, nz, ny, nx): nu_t, dsigma; None detaches."""
+        if clo is None:
+            _check(self.lib.tslb_set_closure_fields(self._h, None), "tslb_set_closure_fields")
+            return
+        a = np.ascontiguousarray(clo, dtype=np.float32)
+        if a.size != 7 * self.n:
+            raise ValueError("closure fields have the wrong size")
+        _check(self.lib.tslb_set_closure_fields(self._h, _fptr(a)), "tslb_set_closure_fields")
+
+    def download_closure(self):
+        out = np.empty((7,) + self.shape, dtype=np.float32)
+        _check(self.lib.tslb_download_closure(self._h, _fptr(out)), "tslb_download_closure")
+        return out
+
+
+CLOSURE_NPARAM = 9496
+
+
+class Closure:
+    """The learned subgrid closure (tslb_closure_* of include/tslb.h): an MLP
+    10 -> 64 -> 64 -> 64 -> 7 on the matrix cores, its training samples
+    (box-filtered lattice states) and its Adam state, on the current device."""
+
+    def __init__(self, max_samples=0):
+        self.lib = load_library()
+        h = _P()
+        _check(self.lib.tslb_closure_create(ctypes.byref(h), int(max_samples)), "tslb_closure_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.tslb_closure_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, theta):
+        a = np.ascontiguousarray(theta, dtype=np.float32)
+        if a.size != CLOSURE_NPARAM:
+            raise ValueError("expected %d parameters" % CLOSURE_NPARAM)
+        _check(self.lib.tslb_closure_set_params(self._h, _fptr(a), CLOSURE_NPARAM), "tslb_closure_set_params")
+
+    def get_params(self):
+        out = np.empty(CLOSURE_NPARAM, dtype=np.float32)
+        _check(self.lib.tslb_closure_get_params(self._h, _fptr(out), CLOSURE_NPARAM), "tslb_closure_get_params")
+        return out
+
+    def add_samples(self, lattice, factor):
+        _check(self.lib.tslb_closure_add_samples(self._h, lattice._h, int(factor)), "tslb_closure_add_samples")
+
+    @property
+    def num_samples(self):
+        n = ctypes.c_int64()
+        _check(self.lib.tslb_closure_num_samples(self._h, ctypes.byref(n)), "tslb_closure_num_samples")
+        return n.value
+
+    def clear_samples(self):
+        _check(self.lib.tslb_closure_clear_samples(self._h), "tslb_closure_clear_samples")
+
+    def get_samples(self):
+        """(feat (10, ns), aux (7, ns)) of the stored samples."""
+        ns = self.num_samples
+        f = np.empty((10, ns), dtype=np.float32)
+        a = np.empty((7, ns), dtype=np.float32)
+        _check(self.lib.tslb_closure_get_samples(self._h, _fptr(f), _fptr(a)), "tslb_closure_get_samples")
+        return f, a
+
+    def set_samples(self, feat, aux):
+        f = np.ascontiguousarray(feat, dtype=np.float32)
+        a = np.ascontiguousarray(aux, dtype=np.float32)
+        ns = f.shape[1]
+        if f.shape != (10, ns) or a.shape != (7, ns):
+            raise ValueError("feat must be (10, ns) and aux (7, ns)")
+        _check(self.lib.tslb_closure_set_samples(self._h, _fptr(f), _fptr(a), ns), "tslb_closure_set_samples")
+
+    def forward(self):
+        """Outputs (7, ns) of the stored samples: nu_t, dsigma."""
+        out = np.empty((7, self.num_samples), dtype=np.float32)
+        _check(self.lib.tslb_closure_forward(self._h, _fptr(out)), "tslb_closure_forward")
+        return out
+
+    def loss_grad(self, t_scale, idx=None, grad=True):
+        loss = ctypes.c_double()
+        g = np.empty(CLOSURE_NPARAM, dtype=np.float32) if grad else None
+        if idx is None:
+            ip, nb = None, 0
+        else:
+            ia = np.ascontiguousarray(idx, dtype=np.int32)
+            ip, nb = ia.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ia.size
+        _check(self.lib.tslb_closure_loss_grad(self._h, ip, nb, float(t_scale), ctypes.byref(loss),
+                                               _fptr(g) if grad else None), "tslb_closure_loss_grad")
+        return (loss.value, g) if grad else loss.value
+
+    def train(self, steps, batch, lr, t_scale, seed=0):
+        loss = ctypes.c_double()
+        _check(self.lib.tslb_closure_train(self._h, int(steps), int(batch), float(lr), float(t_scale),
+                                           int(seed) & 0xFFFFFFFF, ctypes.byref(loss)), "tslb_closure_train")
+        return loss.value

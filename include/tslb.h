@@ -231,6 +231,63 @@ int tslb_last_error(char *buf, int len);
 /* Library version string, e.g. "tslb 0.1 gfx950". */
 const char *tslb_version(void);
 
+/* ---------------------------------------------------------------------------
+ * Learned subgrid closure.  No counterpart in the reference (whose LES options
+ * stop at the fixed Smagorinsky-free TSLB step): a per-node MLP, 10 inputs
+ * (strain rate Sxx, Syy, Szz, Sxy, Sxz, Syz from the isotropic D3Q27 gradient
+ * of the velocity -- the stencil of the colour gradient,
+ * recursive_2c_TSLB3D.f90:555-566 --, grad(phi), phi) -> 64 -> 64 -> 64 -> 7
+ * (tanh-form GELU), outputs nu_t = s0 softplus(o0) and six stress corrections
+ * dsigma_k = s_{k+1} o_{k+1}.  The collision then relaxes with
+ * omega = 1/(tau + 3 nu_t) and adds dsigma to the post-collision
+ * non-equilibrium stress.  BF16 matrix-core products, FP32 accumulation.
+ *
+ * Parameters (TSLB_CLOSURE_NPARAM floats, natural layout, each layer's weight
+ * in_features x out_features row-major): W1 (10x64), b1, W2 (64x64), b2, W3,
+ * b3, W4 (64x7), b4, in_scale (10; inputs are multiplied by it), out_scale (7).
+ *
+ * Samples for training: tslb_closure_add_samples box-filters a lattice by
+ * factor^3 blocks (rho_bar, Favre velocity, phi_bar) and stores, per coarse
+ * node, the 10 inputs of the filtered field and rho_bar with the subgrid
+ * stress tau_ab = <rho u_a u_b> - rho_bar u~_a u~_b.  The training loss is
+ * mean over samples of sum_k c_k (M_k - tau_k)^2 / t_scale^2 with
+ * M = -2 rho_bar nu_t S + dsigma, c = (1,1,1,2,2,2); FP64 loss sum.
+ * --------------------------------------------------------------------------- */
+#define TSLB_CLOSURE_NPARAM 9496
+typedef struct tslb_closure tslb_closure;
+/* A model with room for max_samples training samples, on the current device. */
+int tslb_closure_create(tslb_closure **out, int64_t max_samples);
+int tslb_closure_destroy(tslb_closure *m);
+int tslb_closure_set_params(tslb_closure *m, const float *theta, int n);
+int tslb_closure_get_params(tslb_closure *m, float *theta, int n);
+/* Append the samples of the lattice's current state filtered by factor^3
+ * blocks (factor divides nx, ny, nz; ncomp 1 or 2). */
+int tslb_closure_add_samples(tslb_closure *m, tslb_ctx *ctx, int factor);
+int tslb_closure_num_samples(tslb_closure *m, int64_t *ns);
+int tslb_closure_clear_samples(tslb_closure *m);
+/* Stored samples, SoA: feat[10][ns], aux[7][ns] (rho_bar, tau[6]). */
+int tslb_closure_get_samples(tslb_closure *m, float *feat, float *aux);
+int tslb_closure_set_samples(tslb_closure *m, const float *feat, const float *aux, int64_t ns);
+/* Outputs (nu_t, dsigma[6]) of every stored sample, SoA out[7][ns]. */
+int tslb_closure_forward(tslb_closure *m, float *out);
+/* Loss and gradient (TSLB_CLOSURE_NPARAM floats; the scale entries are 0) over
+ * the samples idx[0..nb) (idx NULL: all samples); grad may be NULL. */
+int tslb_closure_loss_grad(tslb_closure *m, const int32_t *idx, int64_t nb, float t_scale, double *loss,
+                           float *grad);
+/* steps Adam steps (beta 0.9, 0.999, eps 1e-8) on random batches of `batch`
+ * samples drawn from (seed, step); *last_loss = loss of the last batch. */
+int tslb_closure_train(tslb_closure *m, int steps, int batch, float lr, float t_scale, uint32_t seed,
+                       double *last_loss);
+/* Attach a model to a lattice (ncomp 1 or 2, no jet boundary): before every
+ * collision the closure fields are recomputed from the current state.  NULL
+ * detaches.  The model must outlive the attachment. */
+int tslb_set_closure(tslb_ctx *ctx, tslb_closure *m);
+/* Fixed closure fields clo[7][n] (nu_t, dsigma) used by every collision; NULL
+ * detaches. */
+int tslb_set_closure_fields(tslb_ctx *ctx, const float *clo);
+/* Closure fields clo[7][n] the next collision uses (recomputed for a model). */
+int tslb_download_closure(tslb_ctx *ctx, float *clo);
+
 #ifdef __cplusplus
 }
 #endif

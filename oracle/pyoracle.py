@@ -51,6 +51,7 @@ def lib():
             getattr(L, f"orc_feq_{sfx}").argtypes = [ct, ct, ct, ct, cp]
             getattr(L, f"orc_aneq_{sfx}").argtypes = [cp, ct, ct, ct, cp]
             getattr(L, f"orc_set_jet_{sfx}").argtypes = [P, ctypes.c_int]
+            getattr(L, f"orc_set_closure_{sfx}").argtypes = [P, cp]
             getattr(L, f"orc_set_colour_{sfx}").argtypes = [P, ctypes.c_double, ctypes.c_double]
             getattr(L, f"orc_colour_values_{sfx}").argtypes = [
                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ct, cp, cp, ct, cp,
@@ -171,6 +172,15 @@ class Oracle:
     def set_jet(self, enable=True):
         """Turbulent-jet inlet (k=0) / outlet (k=nz-1) wall planes; set before init."""
         self._fn("set_jet")(self._h, int(bool(enable)))
+
+    def set_closure(self, clo):
+        """Per-node learned-closure fields (7, nz, ny, nx): nu_t, dsigma; None clears."""
+        if clo is None:
+            self._fn("set_closure")(self._h, None)
+            return
+        a = np.ascontiguousarray(clo, dtype=self.dtype).reshape(-1)
+        assert a.size == 7 * self.n
+        self._fn("set_closure")(self._h, _ptr(a, self.ct))
 
     def init(self, rho, u, v, w, phi=None, prepush=True):
         arrs = [np.ascontiguousarray(x, dtype=self.dtype).reshape(-1) for x in (rho, u, v, w)]

@@ -96,6 +96,7 @@ typedef struct {
   int32_t *isf;       /* 1 fluid, 0 solid */
   real *pops;         /* pull scratch: [27][n] */
   real *gpops;        /* pull scratch: [7][n] */
+  real *clo;          /* learned closure fields nu_t, dsigma[6]: [7][n] (NULL: none) */
 } ctx_t;
 
 /* ---------------------------------------------------------------------- */
@@ -295,7 +296,23 @@ void FN(orc_destroy)(void *h) {
   free(c->gpops);
   free(c->fa);
   free(c->apops);
+  free(c->clo);
   free(c);
+}
+
+/* Learned closure (no reference counterpart): per-node nu_t and stress
+ * correction dsigma; the collision of node p uses omega = 1/(tau + 3 nu_t)
+ * and the post-collision non-equilibrium stress (1 - omega) Pi + dsigma,
+ * which a^neq, linear in Pi, turns into populations.  NULL clears. */
+void FN(orc_set_closure)(void *h, const real *clo) {
+  ctx_t *c = (ctx_t *)h;
+  if (!clo) {
+    free(c->clo);
+    c->clo = NULL;
+    return;
+  }
+  if (!c->clo) c->clo = (real *)malloc(7 * c->n * sizeof(real));
+  memcpy(c->clo, clo, 7 * c->n * sizeof(real));
 }
 
 void FN(orc_set_isfluid)(void *h, const int32_t *isf) {
@@ -474,7 +491,12 @@ static void collide_node(const ctx_t *c, size_t p, const real *shift, real *fo, 
     vs = vs + c->fy + c->lin_a * v;
     ws = ws + c->fz + c->lin_a * w;
   }
-  const real om1 = (real)1 - c->omega;
+  real om1 = (real)1 - c->omega;
+  if (c->clo) {
+    const real om = (real)1 / ((real)1 / c->omega + (real)3 * c->clo[p]);
+    for (int q = 0; q < 6; q++) P[q] = ((real)1 - om) * P[q] + c->clo[(size_t)(1 + q) * c->n + p];
+    om1 = (real)1;
+  }
   for (int l = 0; l < 27; l++) {
     real val = feq_l(l, rho, us, vs, ws) + om1 * aneq_l(l, P, u, v, w) * wgt(l);
     if (c->ncomp == 1 && c->force_mode == 0 && l > 0)
