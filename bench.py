@@ -82,8 +82,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="lattice edge (512 for the named workload)")
-    ap.add_argument("--workload", default="emulsion", choices=("emulsion", "channel", "tgv"),
-                    help="emulsion (2C, the named workload), channel (1C, n x n/2 x n/2), tgv (1C, n^3)")
+    ap.add_argument("--workload", default="emulsion", choices=("emulsion", "channel", "tgv", "closure-les"),
+                    help="emulsion (2C, the named workload), channel (1C, n x n/2 x n/2), tgv (1C, n^3), "
+                         "closure-les (learned closure on n/4 against the filtered n^3 run)")
     ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16", "bf16"),
                     help="moment storage precision (compute is FP32)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -97,6 +98,9 @@ def main():
     from entropicthreadsafe_amd import configs
     from entropicthreadsafe_amd.tslb import Lattice
 
+    if args.workload == "closure-les":
+        closure_les(args)
+        return
     n = args.n
     t0 = time.time()
     if args.workload == "emulsion":
@@ -173,6 +177,39 @@ def main():
         except Exception as e:  # the baseline is reported, never the product
             log(f"bench: cpu baseline failed: {e}")
             out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def closure_les(args):
+    """The closure-LES line: wall-time ratio of the fine run to the coarse run
+    with the learned closure over the same physical horizon, with the
+    energy-spectrum errors of the coarse runs (closure and plain) against the
+    filtered fine run."""
+    from entropicthreadsafe_amd import les
+    res = les.run(n=args.n, log=log)
+    out = {
+        "metric": "closure-LES speedup at fixed error",
+        "value": round(res["speedup_closure_vs_fine"], 2),
+        "unit": "x (fine wall time / closure coarse wall time, same physical horizon)",
+        "n_gpus": 1,
+        "steps": res["horizon_fine_steps"],
+        "warmup": res["spinup"],
+        "ms_per_step": round(res["closure_seconds"] / res["horizon_coarse_steps"] * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 lattice, bf16 closure operands",
+        "data": "synthetic (seeded solenoidal field and droplets)",
+        "config": {"workload": res["workload"], "factor": res["factor"], "kmax": res["kmax"],
+                   "train_steps": res["train_steps"], "batch": res["batch"], "samples": res["samples"]},
+        "closure_spectrum_error": res["closure_spectrum_error"],
+        "plain_spectrum_error": res["plain_spectrum_error"],
+        "fine_seconds": round(res["fine_seconds"], 4),
+        "closure_seconds": round(res["closure_seconds"], 4),
+        "plain_seconds": round(res["plain_seconds"], 4),
+        "train_seconds": round(res["train_seconds"], 3),
+        "train_loss": [res["train_loss_initial"], res["train_loss_final"]],
+    }
     print(json.dumps(out), flush=True)
 
 

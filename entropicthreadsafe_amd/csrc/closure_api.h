@@ -54,9 +54,9 @@ __global__ __launch_bounds__(256) void k_sample_idx(int *__restrict__ idx, int n
   x ^= x >> 31;
   idx[i] = (int)(x % (uint64_t)ns);
 }
-__global__ __launch_bounds__(256) void k_iota(int *__restrict__ idx, int nb) {
+__global__ __launch_bounds__(256) void k_iota(int *__restrict__ idx, int nb, int first) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < nb) idx[i] = i;
+  if (i < nb) idx[i] = first + i;
 }
 
 void pack_weights(tslb_closure *m, hipStream_t st) {
@@ -103,12 +103,19 @@ int ensure_train(tslb_closure *m, size_t nb) {
   return TSLB_OK;
 }
 
-// gradient (and FP64 loss) of the samples idx[0..nb) into m->grad, m->loss
-int run_grad(tslb_closure *m, int nb, float t_scale) {
+// batches above this many samples are processed in chunks
+constexpr int64_t kMaxChunk = 1 << 20;
+
+void zero_grad(tslb_closure *m) {
+  hipLaunchKernelGGL(k_fill, dim3((kMlpNParam + 255) / 256), dim3(256), 0, m->stream, m->grad, 0.f, kMlpNParam);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, m->stream, (float *)m->loss, 0.f, 2);
+}
+
+// adds the gradient (and FP64 loss) of the samples idx[0..nb) to m->grad,
+// m->loss; ntot is the batch size the mean is taken over
+int run_grad(tslb_closure *m, int nb, int64_t ntot, float t_scale) {
   hipStream_t st = m->stream;
-  hipLaunchKernelGGL(k_fill, dim3((kMlpNParam + 255) / 256), dim3(256), 0, st, m->grad, 0.f, kMlpNParam);
-  hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, (float *)m->loss, 0.f, 2);
-  const float inv = 1.f / (t_scale * t_scale * (float)nb);
+  const float inv = (float)(1.0 / ((double)t_scale * t_scale * (double)ntot));
   const MlpTrainBufs tb = train_bufs(m);
   hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(grid_for_tiles((nb + 15) / 16)), dim3(256), 0, st, m->feat,
                      (long long)m->cap, m->aux, (long long)m->cap, m->idx, nb, m->ldb, m->wf, m->wb, m->th, inv,
@@ -336,10 +343,21 @@ int tslb_closure_loss_grad(tslb_closure *m, const int32_t *idx, int64_t nb, floa
     for (int64_t i = 0; i < nb; i++)
       if (idx[i] < 0 || (size_t)idx[i] >= m->ns) return fail(TSLB_E_ARG, "sample index out of range");
   }
-  if ((rc = ensure_train(m, (size_t)nb))) return rc;
-  if (idx) HIPCK(hipMemcpyAsync(m->idx, idx, nb * sizeof(int32_t), hipMemcpyHostToDevice, m->stream));
-  else hipLaunchKernelGGL(k_iota, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, m->stream, m->idx, (int)nb);
-  if ((rc = run_grad(m, (int)nb, t_scale))) return rc;
+  const int64_t chunk = nb < kMaxChunk ? nb : kMaxChunk;
+  if ((rc = ensure_train(m, (size_t)chunk))) return rc;
+  zero_grad(m);
+  for (int64_t c0 = 0; c0 < nb; c0 += chunk) {
+    const int len = (int)(nb - c0 < chunk ? nb - c0 : chunk);
+    if (idx) {
+      HIPCK(hipMemcpyAsync(m->idx, idx + c0, len * sizeof(int32_t), hipMemcpyHostToDevice, m->stream));
+    } else {
+      hipLaunchKernelGGL(k_iota, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, m->stream, m->idx, len,
+                         (int)c0);
+    }
+    if ((rc = run_grad(m, len, nb, t_scale))) return rc;
+    // the next chunk's indices overwrite m->idx once this chunk is done
+    if (idx) HIPCK(hipStreamSynchronize(m->stream));
+  }
   HIPCK(hipMemcpyAsync(loss, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream));
   if (grad) HIPCK(hipMemcpyAsync(grad, m->grad, kMlpNParam * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   HIPCK(hipStreamSynchronize(m->stream));
@@ -350,14 +368,15 @@ int tslb_closure_train(tslb_closure *m, int steps, int batch, float lr, float t_
                        double *last_loss) {
   int rc = check_model(m);
   if (rc) return rc;
-  if (steps < 0 || batch <= 0) return fail(TSLB_E_ARG, "steps < 0 or batch <= 0");
+  if (steps < 0 || batch <= 0 || batch > kMaxChunk) return fail(TSLB_E_ARG, "steps < 0 or batch outside 1..2^20");
   if (!m->ns) return fail(TSLB_E_STATE, "no samples");
   if (!(t_scale > 0.f) || !(lr > 0.f)) return fail(TSLB_E_ARG, "lr and t_scale must be positive");
   if ((rc = ensure_train(m, (size_t)batch))) return rc;
   for (int s = 0; s < steps; s++) {
     hipLaunchKernelGGL(k_sample_idx, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, m->stream, m->idx, batch,
                        (long long)m->ns, seed, (uint32_t)(m->adam_t + 1));
-    if ((rc = run_grad(m, batch, t_scale))) return rc;
+    zero_grad(m);
+    if ((rc = run_grad(m, batch, batch, t_scale))) return rc;
     m->adam_t += 1;
     hipLaunchKernelGGL(k_mlp_adam, dim3((kOffIn + 255) / 256), dim3(256), 0, m->stream, m->th, m->grad, m->m1, m->m2,
                        lr, 0.9f, 0.999f, 1e-8f, m->adam_t);

@@ -1312,6 +1312,168 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
   }
 }
 
+// Sparse curvature of the phase field in two passes.  The gate
+// phi(1-phi)|n|^2 > 0.2 with |n| < 1 needs phi(1-phi) > 0.2 at the node
+// itself (0.19 leaves room for rounding), and in the emulsion only a thin
+// shell of tile-planes holds such nodes.
+//
+// k_iface_list: per (tile, plane), a vote over the tile's own nodes; tile-
+// planes that may carry curvature are appended to a list (vector atomic on
+// a counter), every flag byte is cleared.  A 64x8 block marches a z-chunk
+// with the phi plane four ahead in flight.
+template <typename S>
+__global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, uint8_t *__restrict__ flags,
+                                                     int *__restrict__ cnt, int *__restrict__ list, Params p) {
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile_x = b % p.tiles_x;
+  b /= p.tiles_x;
+  const int tile_y = b % p.tiles_y;
+  const int zc = b / p.tiles_y;
+  const int z0 = zc * p.z_chunk;
+  const int z1 = min(z0 + p.z_chunk, p.nz);
+  const int x = tile_x * TX + tx, y = tile_y * TY + ty;
+  const bool active = x < p.nx && y < p.ny;
+  const size_t plane = (size_t)p.nx * p.ny;
+  const int tiles_plane = p.tiles_x * p.tiles_y;
+  const int tile = tile_y * p.tiles_x + tile_x;
+  const uint32_t pbS = (uint32_t)(plane * sizeof(S));
+  const uint32_t off = active ? (uint32_t)(x + p.nx * y) * (uint32_t)sizeof(S) : kOOB;
+  auto ld = [&](int z) -> S {
+    if (z >= z1) return S{};
+    return bld<S>(rsrc(phi + plane * (size_t)z, pbS), off);
+  };
+  S q0 = ld(z0), q1 = ld(z0 + 1), q2 = ld(z0 + 2), q3 = ld(z0 + 3);
+  auto body = [&](int z, S &q) {
+    float f = to_f(q);
+    if constexpr (sizeof(S) != 4) f *= p.isc[TSLB_PHI];
+    q = ld(z + 4);
+    const bool hit = active && f * (1.f - f) > 0.19f;
+    const int any = __syncthreads_or(hit);
+    if (tid == 0) {
+      const int tp = z * tiles_plane + tile;
+      flags[tp] = 0;
+      if (any) list[atomicAdd(cnt, 1)] = tp;
+    }
+  };
+  for (int z = z0; z < z1; z += 4) {
+    body(z, q0);
+    if (z + 1 >= z1) break;
+    body(z + 1, q1);
+    if (z + 2 >= z1) break;
+    body(z + 2, q2);
+    if (z + 3 >= z1) break;
+    body(z + 3, q3);
+  }
+}
+
+// k_curv_list: curvature of the listed tile-planes.  Per entry (tile, z) a
+// block stages phi of planes z-2..z+2 on the (TX+4)x(TY+4) footprint,
+// computes n of planes z-1, z, z+1 on the (TX+2)x(TY+2) footprint and
+// kappa = -div(n) at its own nodes past the gate -- the arithmetic of
+// k_normals_tiled, so the values are the same bits -- and writes kappa and
+// the flag byte when any kappa is nonzero.  Blocks stride over the list
+// (its length read from the counter), so every wave ends its loop.
+template <typename S>
+__global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, float *__restrict__ kap,
+                                                    uint8_t *__restrict__ flags, const int *__restrict__ cnt,
+                                                    const int *__restrict__ list, const uint32_t *__restrict__ links,
+                                                    int has_mask, Params p) {
+  __shared__ float ph[5][NPSLOT];
+  __shared__ float nl[3][3][NSLOT];
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const size_t plane = (size_t)p.nx * p.ny;
+  const int tiles_plane = p.tiles_x * p.tiles_y;
+  const uint32_t pbS = (uint32_t)(plane * sizeof(S)), pb4 = (uint32_t)(plane * 4);
+  const bool two = tid + NTHR < NPSLOT;
+  const bool ntwo = tid + NTHR < NSLOT;
+  const int n_entries = *cnt;
+  for (int e = blockIdx.x; e < n_entries; e += gridDim.x) {
+    const int tp = list[e];
+    const int z = tp / tiles_plane, tile = tp - z * tiles_plane;
+    const int x0 = (tile % p.tiles_x) * TX, y0 = (tile / p.tiles_x) * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = x < p.nx && y < p.ny;
+    auto phi_off = [&](int sl) -> uint32_t {
+      const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
+      const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
+      return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
+    };
+    const uint32_t o0 = phi_off(tid), o1 = two ? phi_off(tid + NTHR) : 0u;
+    S v0[5], v1[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const rsrc_t r = rsrc(phi + plane * (size_t)clampw(z - 2 + k, p.nz, p.pz), pbS);
+      v0[k] = bld<S>(r, o0);
+      if (two) v1[k] = bld<S>(r, o1);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      float a = to_f(v0[k]), c = two ? to_f(v1[k]) : 0.f;
+      if constexpr (sizeof(S) != 4) {
+        a *= p.isc[TSLB_PHI];
+        c *= p.isc[TSLB_PHI];
+      }
+      ph[k][tid] = a;
+      if (two) ph[k][tid + NTHR] = c;
+    }
+    __syncthreads();
+    // n of planes z-1, z, z+1 (nl[0..2]) on the normal footprint
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int zz = z - 1 + k;
+      int gz = 0;
+      const bool zin = src_coord(zz, p.nz, p.pz, gz);
+      const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
+      auto one = [&](int sl) {
+        const int sx = sl % SW, sy = sl / SW;
+        int gx = 0, gy = 0;
+        const bool okxy = src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
+        float a = 0.f, bb = 0.f, cc = 0.f;
+        if (zin && okxy) {
+          bool ok = true;
+          if (has_mask) ok = (bld_u32(lk, (uint32_t)(gx + p.nx * gy) * 4u) & kSolid) == 0u;
+          if (ok) {
+            const int c = (sy + 1) * PW + (sx + 1);
+            auto get = [&](int dx, int dy, int dz) -> float { return ph[k + 1 + dz][c + dy * PW + dx]; };
+            phi_normal(get, a, bb, cc);
+          }
+        }
+        nl[k][0][sl] = a;
+        nl[k][1][sl] = bb;
+        nl[k][2][sl] = cc;
+      };
+      one(tid);
+      if (ntwo) one(tid + NTHR);
+    }
+    __syncthreads();
+    float curv = 0.f;
+    if (active) {
+      const int own = (ty + 1) * SW + (tx + 1);
+      const float a = nl[1][0][own], bb = nl[1][1][own], c = nl[1][2][own];
+      const float f = ph[2][(ty + 2) * PW + (tx + 2)];
+      const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
+      if (ind > 0.2f) {
+        auto nat = [&](int comp) {
+          return [&, comp](int dx, int dy, int dz) -> float {
+            return nl[1 + dz][comp][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+          };
+        };
+        const float dxx = grad_axis<0>(nat(0));
+        const float dyy = grad_axis<1>(nat(1));
+        const float dzz = grad_axis<2>(nat(2));
+        curv = -(dxx + dyy + dzz);
+      }
+    }
+    const int flag = __syncthreads_or(curv != 0.f);
+    if (flag && active) bst(rsrc(kap + plane * (size_t)z, pb4), (uint32_t)(x + p.nx * y) * 4u, curv);
+    if (tid == 0) flags[tp] = (uint8_t)flag;
+    __syncthreads();  // LDS reuse by the next entry
+  }
+}
+
 // plain copy for bandwidth calibration: each thread moves 4 float4 per
 // iteration (16 B per lane per instruction, fully coalesced)
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1716,7 +1878,9 @@ struct tslb_ctx {
   void *buf[2] = {nullptr, nullptr};
   int cur = 0;
   float *nrm = nullptr;  // n0, n1, n2, kappa: 4 * fs floats
-  uint8_t *flags = nullptr;  // per (tile, plane): n / kappa nonzero there
+  uint8_t *flags = nullptr;  // per (tile, plane): kappa (phase field) or gradient nonzero there
+  int *ilist = nullptr;      // phase field: counter, then the tile-planes that may carry curvature
+  int ncu = 256;             // compute units of the device
   uint32_t *links = nullptr;
   int has_mask = 0;
   int normals_fresh = 0;
@@ -1832,11 +1996,21 @@ int launch_normals(tslb_ctx *c, int dense = 0) {
                          field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
                          c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->flags, c->links, c->has_mask, 1,
                          c->kp);
-    else
+    else if (dense)
       hipLaunchKernelGGL((k_normals_tiled<S, false>), dim3(nblk), dim3(NTHR), 0, c->stream,
                          field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI), c->nrm, c->nrm + c->fs,
                          c->nrm + 2 * c->fs, c->nrm + 3 * c->fs, c->flags, c->links, c->has_mask,
-                         dense, c->kp);
+                         1, c->kp);
+    else {
+      // sparse curvature: list the tile-planes that may carry it, then
+      // compute those only
+      const S *phi = field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI);
+      hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->ilist, 0.f, 1);
+      hipLaunchKernelGGL((k_iface_list<S>), dim3(nblk), dim3(NTHR), 0, c->stream, phi, c->flags, c->ilist,
+                         c->ilist + 1, c->kp);
+      hipLaunchKernelGGL((k_curv_list<S>), dim3(4 * c->ncu), dim3(NTHR), 0, c->stream, phi, c->nrm + 3 * c->fs,
+                         c->flags, c->ilist, c->ilist + 1, c->links, c->has_mask, c->kp);
+    }
     return 0;
   });
   HIPCK(hipGetLastError());
@@ -2042,6 +2216,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, p->device) == hipSuccess && prop.multiProcessorCount > 0)
       ncu = prop.multiProcessorCount;
+    c->ncu = ncu;
     const long long tiles = (long long)k.tiles_x * k.tiles_y;
     int zc = 64;
     while (zc > 8 && tiles * ((p->nz + zc - 1) / zc) < 2LL * ncu) zc /= 2;
@@ -2076,6 +2251,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
       (p->ncomp >= 2 && (rc = dev_alloc(c, (void **)&c->nrm, 4 * c->fs * sizeof(float)))) ||
       (p->ncomp >= 2 &&
        (rc = dev_alloc(c, (void **)&c->flags, (size_t)k.tiles_x * k.tiles_y * p->nz))) ||
+      (p->ncomp == 2 &&
+       (rc = dev_alloc(c, (void **)&c->ilist, ((size_t)k.tiles_x * k.tiles_y * p->nz + 1) * sizeof(int)))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
       (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
     std::string msg = g_err;
@@ -2112,6 +2289,7 @@ int tslb_destroy(tslb_ctx *c) {
   (void)hipFree(c->buf[1]);
   (void)hipFree(c->nrm);
   (void)hipFree(c->flags);
+  (void)hipFree(c->ilist);
   (void)hipFree(c->links);
   (void)hipFree(c->part);
   (void)hipFree(c->red);

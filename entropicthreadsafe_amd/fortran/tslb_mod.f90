@@ -106,6 +106,40 @@ module tslb_mod
       character(kind=c_char), intent(in) :: path(*)
       integer(c_int64_t), intent(out) :: step
     end function
+    ! learned subgrid closure (no counterpart in the reference drivers)
+    integer(c_int) function tslb_closure_create(m, max_samples) bind(C, name='tslb_closure_create')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), intent(out) :: m
+      integer(c_int64_t), value :: max_samples
+    end function
+    integer(c_int) function tslb_closure_destroy(m) bind(C, name='tslb_closure_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function tslb_closure_set_params(m, theta, n) bind(C, name='tslb_closure_set_params')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(in) :: theta(*)
+      integer(c_int), value :: n
+    end function
+    integer(c_int) function tslb_closure_add_samples(m, ctx, factor) bind(C, name='tslb_closure_add_samples')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m, ctx
+      integer(c_int), value :: factor
+    end function
+    integer(c_int) function tslb_closure_train(m, steps, batch, lr, t_scale, seed, last_loss) &
+        bind(C, name='tslb_closure_train')
+      import :: c_int, c_ptr, c_float, c_int32_t, c_double
+      type(c_ptr), value :: m
+      integer(c_int), value :: steps, batch
+      real(c_float), value :: lr, t_scale
+      integer(c_int32_t), value :: seed
+      real(c_double), intent(out) :: last_loss
+    end function
+    integer(c_int) function tslb_set_closure(ctx, m) bind(C, name='tslb_set_closure')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, m
+    end function
     integer(c_int) function tslb_reduce(ctx, out) bind(C, name='tslb_reduce')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

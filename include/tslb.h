@@ -275,7 +275,8 @@ int tslb_closure_forward(tslb_closure *m, float *out);
 int tslb_closure_loss_grad(tslb_closure *m, const int32_t *idx, int64_t nb, float t_scale, double *loss,
                            float *grad);
 /* steps Adam steps (beta 0.9, 0.999, eps 1e-8) on random batches of `batch`
- * samples drawn from (seed, step); *last_loss = loss of the last batch. */
+ * (1 .. 2^20) samples drawn from (seed, step); *last_loss = loss of the last
+ * batch.  The Adam state carries over between calls until set_params. */
 int tslb_closure_train(tslb_closure *m, int steps, int batch, float lr, float t_scale, uint32_t seed,
                        double *last_loss);
 /* Attach a model to a lattice (ncomp 1 or 2, no jet boundary): before every
