@@ -48,9 +48,12 @@ __device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
 __host__ __device__ constexpr int mlp_perm(int ks, int g, int i) {
   return 16 * (2 * ks + (i >> 2)) + 4 * g + (i & 3);
 }
+// z sigmoid(2u) = z / (1 + 2^(-2 u log2 e)), u = c (z + a z^3): one v_exp_f32
+// and one v_rcp_f32
 __device__ __forceinline__ float gelu_f(float z) {
-  const float u = kGeluC * fmaf(kGeluA * z * z, z, z);
-  return z / (1.f + __expf(-2.f * u));
+  constexpr float c0 = -2.f * kGeluC * 1.4426950408889634f, c1 = c0 * kGeluA;
+  const float e = __builtin_amdgcn_exp2f(z * fmaf(c1, z * z, c0));
+  return z * __builtin_amdgcn_rcpf(1.f + e);
 }
 __device__ __forceinline__ float gelu_grad_f(float z) {
   const float u = kGeluC * fmaf(kGeluA * z * z, z, z);
@@ -116,6 +119,28 @@ __device__ __forceinline__ bf16x8 mlp_input(const float *__restrict__ feat, long
   }
   return B;
 }
+// the same in two halves: raw inputs of a tile loaded ahead of their use
+struct MlpRaw {
+  float v[8];
+};
+__device__ __forceinline__ void mlp_load(MlpRaw &r, const float *__restrict__ feat, long long fstride,
+                                         long long node, bool ok, int g) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int k = 8 * g + i;
+    r.v[i] = (k < kMlpNF && ok) ? feat[k * fstride + node] : 0.f;
+  }
+}
+__device__ __forceinline__ bf16x8 mlp_scale(const MlpRaw &r, int g, const float *__restrict__ th) {
+  bf16x8 B;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int k = 8 * g + i;
+    const float v = k < kMlpNF ? r.v[i] * th[kOffIn + k] : (k == kMlpNF ? 1.f : 0.f);
+    B[i] = (__bf16)v;
+  }
+  return B;
+}
 
 // GELU of four accumulator blocks, packed as the two K steps of the next layer
 __device__ __forceinline__ void mlp_act(const f32x4 (&z)[4], bf16x8 (&B)[2]) {
@@ -164,10 +189,16 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ fea
   for (int b = 0; b < kFwdBlocks; b++) A[b] = wf[b * 64 + lane];
   const long long ntiles = (n + 15) / 16;
   const long long nw = (long long)gridDim.x * 4;
-  for (long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += nw) {
+  long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  MlpRaw nxt;
+  mlp_load(nxt, feat, fstride, t * 16 + c, t < ntiles && t * 16 + c < n, g);
+  for (; t < ntiles; t += nw) {
     const long long node = t * 16 + c;
     const bool ok = node < n;
-    const bf16x8 B1 = mlp_input(feat, fstride, node, ok, g, th);
+    const MlpRaw cur = nxt;  // the next tile's inputs load while this one computes
+    const long long tn = t + nw, nn = tn * 16 + c;
+    mlp_load(nxt, feat, fstride, nn, tn < ntiles && nn < n, g);
+    const bf16x8 B1 = mlp_scale(cur, g, th);
     f32x4 z[4];
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

@@ -1317,13 +1317,19 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
 // itself (0.19 leaves room for rounding), and in the emulsion only a thin
 // shell of tile-planes holds such nodes.
 //
-// k_iface_list: per (tile, plane), a vote over the tile's own nodes; tile-
-// planes that may carry curvature are appended to a list (vector atomic on
-// a counter), every flag byte is cleared.  A 64x8 block marches a z-chunk
-// with the phi plane four ahead in flight.
+// k_iface_list: per (tile, plane), a vote over the tile's own nodes; runs of
+// consecutive tile-planes that may carry curvature are appended to a list,
+// every flag byte of the chunk is cleared.  A 64x8 block marches a z-chunk
+// (at most 64 planes) with the phi plane four ahead in flight; each wave
+// keeps a 64-bit mask of its planes with a hit (a ballot, no barrier), the
+// block ORs the eight masks once at the end and appends its runs with one
+// atomic on the counter (a vector atomic).  Entry: 64 (z_first * tiles per
+// plane + tile) + run length - 1.
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, uint8_t *__restrict__ flags,
                                                      int *__restrict__ cnt, int *__restrict__ list, Params p) {
+  __shared__ uint64_t wmask[NTHR / 64];
+  __shared__ int base;
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   int b = xcd_tile(blockIdx.x, gridDim.x);
@@ -1344,18 +1350,14 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
     if (z >= z1) return S{};
     return bld<S>(rsrc(phi + plane * (size_t)z, pbS), off);
   };
+  uint64_t mask = 0;  // wave-uniform: bit k for plane z0 + k
   S q0 = ld(z0), q1 = ld(z0 + 1), q2 = ld(z0 + 2), q3 = ld(z0 + 3);
   auto body = [&](int z, S &q) {
     float f = to_f(q);
     if constexpr (sizeof(S) != 4) f *= p.isc[TSLB_PHI];
     q = ld(z + 4);
     const bool hit = active && f * (1.f - f) > 0.19f;
-    const int any = __syncthreads_or(hit);
-    if (tid == 0) {
-      const int tp = z * tiles_plane + tile;
-      flags[tp] = 0;
-      if (any) list[atomicAdd(cnt, 1)] = tp;
-    }
+    if (__ballot(hit) != 0ull) mask |= 1ull << (z - z0);
   };
   for (int z = z0; z < z1; z += 4) {
     body(z, q0);
@@ -1366,15 +1368,33 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
     if (z + 3 >= z1) break;
     body(z + 3, q3);
   }
+  if ((tid & 63) == 0) wmask[tid >> 6] = mask;
+  if (tid < z1 - z0) flags[(z0 + tid) * tiles_plane + tile] = 0;
+  __syncthreads();
+  uint64_t all = 0;
+#pragma unroll
+  for (int w = 0; w < NTHR / 64; w++) all |= wmask[w];
+  // run starts: set bits whose lower neighbour is clear
+  const uint64_t starts = all & ~(all << 1);
+  if (tid == 0) base = starts ? atomicAdd(cnt, __popcll(starts)) : 0;
+  __syncthreads();
+  // thread k appends the run starting at plane z0 + k, at its rank among the starts
+  if (tid < z1 - z0 && ((starts >> tid) & 1ull)) {
+    const uint64_t above = ~(all >> tid);  // first clear bit at or above k ends the run
+    const int len = above ? __ffsll((long long)above) - 1 : 64 - tid;
+    list[base + __popcll(starts & ((1ull << tid) - 1ull))] = ((z0 + tid) * tiles_plane + tile) * 64 + len - 1;
+  }
 }
 
-// k_curv_list: curvature of the listed tile-planes.  Per entry (tile, z) a
-// block stages phi of planes z-2..z+2 on the (TX+4)x(TY+4) footprint,
-// computes n of planes z-1, z, z+1 on the (TX+2)x(TY+2) footprint and
-// kappa = -div(n) at its own nodes past the gate -- the arithmetic of
-// k_normals_tiled, so the values are the same bits -- and writes kappa and
-// the flag byte when any kappa is nonzero.  Blocks stride over the list
-// (its length read from the counter), so every wave ends its loop.
+// k_curv_list: curvature of the listed runs of tile-planes.  Per run a block
+// marches its planes as k_normals_tiled does: phi of planes z-2..z+2 on the
+// (TX+4)x(TY+4) footprint in a five-plane LDS ring (the plane three ahead
+// loads into registers meanwhile), n of planes z-1..z+1 on the (TX+2)x(TY+2)
+// footprint in a three-plane ring, kappa = -div(n) at the own nodes past the
+// gate -- the arithmetic of k_normals_tiled, so the values are the same bits
+// -- and kappa plus the flag byte written when any kappa of the plane is
+// nonzero.  Blocks stride over the list (its length read from the counter),
+// so every wave ends its loop.
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, float *__restrict__ kap,
                                                     uint8_t *__restrict__ flags, const int *__restrict__ cnt,
@@ -1391,8 +1411,9 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
   const bool ntwo = tid + NTHR < NSLOT;
   const int n_entries = *cnt;
   for (int e = blockIdx.x; e < n_entries; e += gridDim.x) {
-    const int tp = list[e];
-    const int z = tp / tiles_plane, tile = tp - z * tiles_plane;
+    const int ent = list[e];
+    const int len = (ent & 63) + 1, tp0 = ent >> 6;
+    const int zs = tp0 / tiles_plane, tile = tp0 - zs * tiles_plane;
     const int x0 = (tile % p.tiles_x) * TX, y0 = (tile / p.tiles_x) * TY;
     const int x = x0 + tx, y = y0 + ty;
     const bool active = x < p.nx && y < p.ny;
@@ -1402,31 +1423,30 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
     };
     const uint32_t o0 = phi_off(tid), o1 = two ? phi_off(tid + NTHR) : 0u;
-    S v0[5], v1[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      const rsrc_t r = rsrc(phi + plane * (size_t)clampw(z - 2 + k, p.nz, p.pz), pbS);
-      v0[k] = bld<S>(r, o0);
-      if (two) v1[k] = bld<S>(r, o1);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      float a = to_f(v0[k]), c = two ? to_f(v1[k]) : 0.f;
+    auto r5 = [zs](int z) { return (z - zs + 10) % 5; };
+    auto r3 = [zs](int z) { return (z - zs + 3) % 3; };
+    S qa{}, qb{};
+    auto issue = [&](int z) {
+      const rsrc_t r = rsrc(phi + plane * (size_t)clampw(z, p.nz, p.pz), pbS);
+      qa = bld<S>(r, o0);
+      if (two) qb = bld<S>(r, o1);
+    };
+    auto commit = [&](int z) {
+      float a = to_f(qa), c = two ? to_f(qb) : 0.f;
       if constexpr (sizeof(S) != 4) {
         a *= p.isc[TSLB_PHI];
         c *= p.isc[TSLB_PHI];
       }
-      ph[k][tid] = a;
-      if (two) ph[k][tid + NTHR] = c;
-    }
-    __syncthreads();
-    // n of planes z-1, z, z+1 (nl[0..2]) on the normal footprint
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const int zz = z - 1 + k;
+      ph[r5(z)][tid] = a;
+      if (two) ph[r5(z)][tid + NTHR] = c;
+    };
+    // n of plane zz from phi planes zz-1..zz+1
+    auto normals = [&](int zz) {
       int gz = 0;
       const bool zin = src_coord(zz, p.nz, p.pz, gz);
       const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
+      const float *P0 = ph[r5(zz - 1)], *P1 = ph[r5(zz)], *P2 = ph[r5(zz + 1)];
+      float(*dn)[NSLOT] = nl[r3(zz)];
       auto one = [&](int sl) {
         const int sx = sl % SW, sy = sl / SW;
         int gx = 0, gy = 0;
@@ -1437,39 +1457,62 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
           if (has_mask) ok = (bld_u32(lk, (uint32_t)(gx + p.nx * gy) * 4u) & kSolid) == 0u;
           if (ok) {
             const int c = (sy + 1) * PW + (sx + 1);
-            auto get = [&](int dx, int dy, int dz) -> float { return ph[k + 1 + dz][c + dy * PW + dx]; };
+            auto get = [&](int dx, int dy, int dz) -> float {
+              const float *pl = dz < 0 ? P0 : (dz > 0 ? P2 : P1);
+              return pl[c + dy * PW + dx];
+            };
             phi_normal(get, a, bb, cc);
           }
         }
-        nl[k][0][sl] = a;
-        nl[k][1][sl] = bb;
-        nl[k][2][sl] = cc;
+        dn[0][sl] = a;
+        dn[1][sl] = bb;
+        dn[2][sl] = cc;
       };
       one(tid);
       if (ntwo) one(tid + NTHR);
+    };
+    // prologue: phi planes zs-2..zs+2, n of planes zs-1 and zs
+    for (int k = 0; k < 5; k++) {
+      issue(zs - 2 + k);
+      commit(zs - 2 + k);
     }
+    if (len > 1) issue(zs + 3);
     __syncthreads();
-    float curv = 0.f;
-    if (active) {
-      const int own = (ty + 1) * SW + (tx + 1);
-      const float a = nl[1][0][own], bb = nl[1][1][own], c = nl[1][2][own];
-      const float f = ph[2][(ty + 2) * PW + (tx + 2)];
-      const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
-      if (ind > 0.2f) {
-        auto nat = [&](int comp) {
-          return [&, comp](int dx, int dy, int dz) -> float {
-            return nl[1 + dz][comp][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+    normals(zs - 1);
+    normals(zs);
+    for (int z = zs; z < zs + len; z++) {
+      normals(z + 1);
+      __syncthreads();
+      float curv = 0.f;
+      if (active) {
+        const int own = (ty + 1) * SW + (tx + 1);
+        const float(*n0)[NSLOT] = nl[r3(z)];
+        const float a = n0[0][own], bb = n0[1][own], c = n0[2][own];
+        const float f = ph[r5(z)][(ty + 2) * PW + (tx + 2)];
+        const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
+        if (ind > 0.2f) {
+          const int zm = r3(z - 1), z0s = r3(z), zp = r3(z + 1);
+          auto nat = [&](int comp) {
+            return [&, comp](int dx, int dy, int dz) -> float {
+              const int zz = dz < 0 ? zm : (dz > 0 ? zp : z0s);
+              return nl[zz][comp][(ty + 1 + dy) * SW + (tx + 1 + dx)];
+            };
           };
-        };
-        const float dxx = grad_axis<0>(nat(0));
-        const float dyy = grad_axis<1>(nat(1));
-        const float dzz = grad_axis<2>(nat(2));
-        curv = -(dxx + dyy + dzz);
+          const float dxx = grad_axis<0>(nat(0));
+          const float dyy = grad_axis<1>(nat(1));
+          const float dzz = grad_axis<2>(nat(2));
+          curv = -(dxx + dyy + dzz);
+        }
       }
+      // phi of plane z+3 replaces z-2 (read last by n of plane z-1)
+      if (z + 1 < zs + len) {
+        commit(z + 3);
+        if (z + 2 < zs + len) issue(z + 4);
+      }
+      const int flag = __syncthreads_or(curv != 0.f);
+      if (flag && active) bst(rsrc(kap + plane * (size_t)z, pb4), (uint32_t)(x + p.nx * y) * 4u, curv);
+      if (tid == 0) flags[(size_t)z * tiles_plane + tile] = (uint8_t)flag;
     }
-    const int flag = __syncthreads_or(curv != 0.f);
-    if (flag && active) bst(rsrc(kap + plane * (size_t)z, pb4), (uint32_t)(x + p.nx * y) * 4u, curv);
-    if (tid == 0) flags[tp] = (uint8_t)flag;
     __syncthreads();  // LDS reuse by the next entry
   }
 }
@@ -2166,6 +2209,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   // kernels address one z-plane of a field through a buffer descriptor with
   // 32-bit byte offsets
   if ((long long)p->nx * p->ny * 4 > 0x7fffffffLL) return fail(TSLB_E_ARG, "nx*ny*4 bytes must fit in int32");
+  if (p->ncomp == 2 && (long long)((p->nx + TX - 1) / TX) * ((p->ny + TY - 1) / TY) * p->nz * 64 > 0x7fffffffLL)
+    return fail(TSLB_E_ARG, "too many tile-planes for the curvature run list");
   if (p->ncomp != 1 && p->ncomp != 2 && p->ncomp != TSLB_COLOUR)
     return fail(TSLB_E_ARG, "ncomp must be 1, 2 or TSLB_COLOUR");
   if (p->ncomp == TSLB_COLOUR && p->tau_b > 0.f && !(p->tau_b > 0.5f))
@@ -2222,6 +2267,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     while (zc > 8 && tiles * ((p->nz + zc - 1) / zc) < 2LL * ncu) zc /= 2;
     if (const char *v = getenv("TSLB_ZCHUNK")) zc = atoi(v);
     if (zc < 1) zc = 1;
+    if (zc > 64) zc = 64;  // k_iface_list keeps one bit per plane of a chunk
     if (zc > p->nz) zc = p->nz;
     k.z_chunk = zc;
   }
