@@ -150,19 +150,22 @@ __device__ __forceinline__ void mlp_act(const f32x4 (&z)[4], bf16x8 (&B)[2]) {
     for (int i = 0; i < 8; i++) B[ks][i] = (__bf16)gelu_f(z[2 * ks + (i >> 2)][i & 3]);
 }
 
-// one hidden layer (64 -> 64): packed weights A[8] (mb-major, two K steps)
+// one hidden layer (64 -> 64): packed weights, block k at A[k * ST] (mb-major,
+// two K steps; ST = 1 for a register array, 64 for the LDS copy)
+template <int ST = 1>
 __device__ __forceinline__ void mlp_hidden(const bf16x8 *A, const bf16x8 (&B)[2],
                                            const float *__restrict__ bias, int g, f32x4 (&z)[4]) {
 #pragma unroll
   for (int mb = 0; mb < 4; mb++) {
     f32x4 acc = *(const f32x4 *)(bias + 16 * mb + 4 * g);
-    acc = mfma16(A[2 * mb], B[0], acc);
-    acc = mfma16(A[2 * mb + 1], B[1], acc);
+    acc = mfma16(A[(2 * mb) * ST], B[0], acc);
+    acc = mfma16(A[(2 * mb + 1) * ST], B[1], acc);
     z[mb] = acc;
   }
 }
 
 // output layer (64 -> 7 of 16 rows): lane group g holds outputs 4g..4g+3
+template <int ST = 1>
 __device__ __forceinline__ f32x4 mlp_out(const bf16x8 *A, const bf16x8 (&B)[2],
                                          const float *__restrict__ th, int g) {
   f32x4 acc;
@@ -172,21 +175,24 @@ __device__ __forceinline__ f32x4 mlp_out(const bf16x8 *A, const bf16x8 (&B)[2],
     acc[i] = q < kMlpNO ? th[kOffB4 + q] : 0.f;
   }
   acc = mfma16(A[0], B[0], acc);
-  acc = mfma16(A[1], B[1], acc);
+  acc = mfma16(A[ST], B[1], acc);
   return acc;
 }
 
 // Inference: features (SoA, 10 x fstride) of n nodes -> closure fields
 // (SoA, 7 x cstride): nu_t = s0 softplus(o0), dsigma_k = s_{k+1} o_{k+1}.
-// One wave per 16-node tile; the forward weights stay in registers.
+// One wave per 16-node tile.  The forward weight blocks (22.5 KiB) sit in LDS,
+// one copy per block, read as 16-byte operands per MFMA; registers stay free
+// for more waves per SIMD, which hide the MFMA -> GELU -> MFMA chain.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ feat, long long fstride,
                                                    long long n, const bf16x8 *__restrict__ wf,
                                                    const float *__restrict__ th, float *__restrict__ clo,
                                                    long long cstride) {
+  __shared__ bf16x8 wl[kFwdBlocks * 64];
+  for (int i = threadIdx.x; i < kFwdBlocks * 64; i += 256) wl[i] = wf[i];
+  __syncthreads();
   const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-  bf16x8 A[kFwdBlocks];
-#pragma unroll
-  for (int b = 0; b < kFwdBlocks; b++) A[b] = wf[b * 64 + lane];
+  const bf16x8 *A = wl + lane;  // block b at A[b * 64]
   const long long ntiles = (n + 15) / 16;
   const long long nw = (long long)gridDim.x * 4;
   long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -195,6 +201,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ fea
   for (; t < ntiles; t += nw) {
     const long long node = t * 16 + c;
     const bool ok = node < n;
+    // keep the LDS weight reads inside the loop (hoisted, they would occupy
+    // the registers the LDS copy is meant to free)
+    asm volatile("" ::: "memory");
     const MlpRaw cur = nxt;  // the next tile's inputs load while this one computes
     const long long tn = t + nw, nn = tn * 16 + c;
     mlp_load(nxt, feat, fstride, nn, tn < ntiles && nn < n, g);
@@ -202,14 +211,14 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ fea
     f32x4 z[4];
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int mb = 0; mb < 4; mb++) z[mb] = mfma16(A[mb], B1, zero);
+    for (int mb = 0; mb < 4; mb++) z[mb] = mfma16(A[mb * 64], B1, zero);
     bf16x8 B[2];
     mlp_act(z, B);
-    mlp_hidden(A + 4, B, th + kOffB2, g, z);
+    mlp_hidden<64>(A + 4 * 64, B, th + kOffB2, g, z);
     mlp_act(z, B);
-    mlp_hidden(A + 12, B, th + kOffB3, g, z);
+    mlp_hidden<64>(A + 12 * 64, B, th + kOffB3, g, z);
     mlp_act(z, B);
-    const f32x4 o = mlp_out(A + 20, B, th, g);
+    const f32x4 o = mlp_out<64>(A + 20 * 64, B, th, g);
     if (ok && g < 2) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {

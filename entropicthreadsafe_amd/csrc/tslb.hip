@@ -1920,6 +1920,7 @@ struct tslb_ctx {
   size_t es = 4;         // bytes per stored moment value
   void *buf[2] = {nullptr, nullptr};
   int cur = 0;
+  int phi_buf = 0;  // one component: the buffer that holds phi
   float *nrm = nullptr;  // n0, n1, n2, kappa: 4 * fs floats
   uint8_t *flags = nullptr;  // per (tile, plane): kappa (phase field) or gradient nonzero there
   int *ilist = nullptr;      // phase field: counter, then the tile-planes that may carry curvature
@@ -2151,12 +2152,7 @@ int launch_fused(tslb_ctx *c) {
     });
     HIPCK(hipGetLastError());
   }
-  if (p.ncomp == 1) {
-    // phi is carried unchanged by one-component runs
-    HIPCK(hipMemcpyAsync((char *)c->buf[1 - c->cur] + TSLB_PHI * c->fs * c->es,
-                         (char *)c->buf[c->cur] + TSLB_PHI * c->fs * c->es, c->n * c->es,
-                         hipMemcpyDeviceToDevice, c->stream));
-  }
+  // one component: phi stays in buf[phi_buf] (check_ctx moves it when needed)
   c->cur = 1 - c->cur;
   c->normals_fresh = 0;
   return TSLB_OK;
@@ -2176,9 +2172,23 @@ int launch_zbc(tslb_ctx *c) {
   return TSLB_OK;
 }
 
-int check_ctx(tslb_ctx *c) {
+// One-component steps leave phi where it was (the fused pass neither reads
+// nor writes it); every other entry point first brings it into the current
+// buffer, since they read it there or use the other buffer as scratch.
+int check_ctx_step(tslb_ctx *c) {
   if (!c) return fail(TSLB_E_ARG, "null context");
   HIPCK(hipSetDevice(c->prm.device));
+  return TSLB_OK;
+}
+int check_ctx(tslb_ctx *c) {
+  int rc = check_ctx_step(c);
+  if (rc) return rc;
+  if (c->prm.ncomp == 1 && c->phi_buf != c->cur) {
+    HIPCK(hipMemcpyAsync((char *)c->buf[c->cur] + TSLB_PHI * c->fs * c->es,
+                         (char *)c->buf[c->phi_buf] + TSLB_PHI * c->fs * c->es, c->n * c->es,
+                         hipMemcpyDeviceToDevice, c->stream));
+    c->phi_buf = c->cur;
+  }
   return TSLB_OK;
 }
 
@@ -2566,14 +2576,14 @@ int tslb_init_equilibrium(tslb_ctx *c, int prepush) {
 }
 
 int tslb_normals(tslb_ctx *c) {
-  int rc = check_ctx(c);
+  int rc = check_ctx_step(c);
   if (rc) return rc;
   if (c->prm.ncomp < 2) return TSLB_OK;
   return launch_normals(c);
 }
 
 int tslb_collide_stream(tslb_ctx *c) {
-  int rc = check_ctx(c);
+  int rc = check_ctx_step(c);
   if (rc) return rc;
   if (c->prm.ncomp >= 2 && !c->normals_fresh) return fail(TSLB_E_STATE, "tslb_normals() must precede tslb_collide_stream()");
   if (c->clo_mode == 2 && (rc = launch_closure(c))) return rc;
@@ -2581,7 +2591,7 @@ int tslb_collide_stream(tslb_ctx *c) {
 }
 
 int tslb_step(tslb_ctx *c, int nsteps) {
-  int rc = check_ctx(c);
+  int rc = check_ctx_step(c);
   if (rc) return rc;
   if (nsteps < 0) return fail(TSLB_E_ARG, "nsteps < 0");
   for (int s = 0; s < nsteps; s++) {
