@@ -120,7 +120,7 @@ int run_grad(tslb_closure *m, int nb, int64_t ntot, float t_scale) {
   hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(grid_for_tiles((nb + 15) / 16)), dim3(256), 0, st, m->feat,
                      (long long)m->cap, m->aux, (long long)m->cap, m->idx, nb, m->ldb, m->wf, m->wb, m->th, inv,
                      tb, m->loss);
-  const int ksl = 2048;
+  const int ksl = 512;  // columns per weight-gradient slice (multiple of 32)
   const int nslice = (nb + ksl - 1) / ksl;
   const int waves = kWgradJobs * nslice;
   hipLaunchKernelGGL(k_mlp_wgrad, dim3((waves + 3) / 4), dim3(256), 0, st, tb, nb, m->ldb, ksl, m->grad);

@@ -413,12 +413,18 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(MlpTrainBufs tb, int nb, int 
   for (int i = 0; i < 8; i++) ones[i] = (__bf16)(c == 0 ? 1.f : 0.f);
   for (int k = k0; k < k1; k += 32) {
     bf16x8 a, b;
+    if (k + 32 <= k1) {
+      // whole K step: 16-byte loads (rows start at multiples of 32 columns)
+      a = *(const bf16x8 *)(drow + k + 8 * g);
+      b = *(const bf16x8 *)(hrow + k + 8 * g);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const int col = k + 8 * g + i;
-      const bool in = col < k1;
-      a[i] = in ? drow[col] : (__bf16)0.f;
-      b[i] = in ? hrow[col] : (__bf16)0.f;
+      for (int i = 0; i < 8; i++) {
+        const int col = k + 8 * g + i;
+        const bool in = col < k1;
+        a[i] = in ? drow[col] : (__bf16)0.f;
+        b[i] = in ? hrow[col] : (__bf16)0.f;
+      }
     }
     acc = mfma16(a, jb.nb < 0 ? ones : b, acc);
   }

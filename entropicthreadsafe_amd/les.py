@@ -151,8 +151,9 @@ def run(n=512, r=4, k0=4.0, urms=0.04, spinup=1500, snaps=4, snap_every=250, hor
         # training: inputs scaled to unit spread, outputs to the target size
         feat, aux = model.get_samples()
         fstd = feat.std(axis=1) + 1e-12
-        t_scale = float(np.sqrt((aux[1:] ** 2).mean()))
-        theta0 = _init_theta(seed, 1.0 / fstd, t_scale)
+        t_rms = float(np.sqrt((aux[1:] ** 2).mean()))
+        t_scale = t_rms or 1.0   # r = 1: no subgrid stress to fit
+        theta0 = _init_theta(seed, 1.0 / fstd, t_rms or 1e-9)
         model.set_params(theta0)
         l0 = model.loss_grad(t_scale, grad=False)
         t0 = time.perf_counter()
