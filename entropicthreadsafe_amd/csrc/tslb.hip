@@ -457,15 +457,34 @@ struct PlaneSrc {
 // kOOB where the tile-plane has no interface (the buffer then returns 0
 // without a memory access).
 constexpr uint32_t kOOB = 0x80000000u;
-template <int NC, typename S, bool CLO>
-__device__ __forceinline__ void load_src(const PlaneSrc &ps, uint32_t offS, uint32_t off4, uint32_t offC,
-                                         const Params &p, Src<NC, CLO> &s) {
+// Reduced storage: the raw values of one source, loaded before any is
+// converted (left to itself the scheduler reuses one register per load and
+// waits out each round trip).
+template <int NC, typename S>
+struct RawSrc {
+  S v[NC == 3 ? 11 : 10];
+};
+template <int NC, typename S>
+__device__ __forceinline__ void load_raw(const PlaneSrc &ps, uint32_t offS, RawSrc<NC, S> &r) {
 #pragma unroll
-  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) {  // phase field: phi from the LDS ring
-    const float v = to_f(bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS));
-    if constexpr (sizeof(S) == 4) s.m[q] = v;
-    else s.m[q] = v * p.isc[q];
-  }
+  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) r.v[q] = bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
+}
+template <int NC, typename S, bool CLO>
+__device__ __forceinline__ void convert_raw(const RawSrc<NC, S> &r, const Params &p, Src<NC, CLO> &s) {
+#pragma unroll
+  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) s.m[q] = to_f(r.v[q]) * p.isc[q];
+}
+
+// FP32 storage: the moments of one source straight into place.
+template <int NC, bool CLO>
+__device__ __forceinline__ void load_m32(const PlaneSrc &ps, uint32_t offS, Src<NC, CLO> &s) {
+#pragma unroll
+  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) s.m[q] = bld<float>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
+}
+
+// The other loads of one source: colour gradient, gated curvature, closure.
+template <int NC, bool CLO>
+__device__ __forceinline__ void load_extra(const PlaneSrc &ps, uint32_t off4, uint32_t offC, Src<NC, CLO> &s) {
   if constexpr (NC == 3) {  // colour gradient
     s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
     s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
@@ -844,19 +863,38 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     ps.fs4 = fs4;
     const rsrc_t fl = rsrc(flags + tiles_plane * (size_t)gz, (uint32_t)tiles_plane);
     const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
-    ook = zok && oxy;
-    if (ook) {
-      if constexpr (MASK) ook = (bld_u32(lk, o_xy * 4u) & kSolid) == 0u;
-      bool un = NC != 2;  // flags gate the phase-field curvature only
-      if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)o_tile) != 0;
-      if (ook) load_src<NC, S>(ps, o_xy * (uint32_t)sizeof(S), un ? o_xy * 4u : kOOB, o_xy * 4u, p, o);
+    // Issue order: link masks and curvature flags of both sources, then
+    // both sources' moments (a solid source's too: they are in range and
+    // then unused), then what depends on the flags -- so no load waits on
+    // another before the moments are in flight.
+    const bool oin = zok && oxy, rin = zok && rxy;
+    uint32_t olk = 0, rlk = 0;
+    uint32_t ofl = 1, rfl = 1;  // flags gate the phase-field curvature only
+    if constexpr (MASK) {
+      if (oin) olk = bld_u32(lk, o_xy * 4u);
+      if (rin) rlk = bld_u32(lk, r_xy * 4u);
     }
-    rok = zok && rxy;
-    if (rok) {
-      if constexpr (MASK) rok = (bld_u32(lk, r_xy * 4u) & kSolid) == 0u;
-      bool un = NC != 2;  // flags gate the phase-field curvature only
-      if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)r_tile) != 0;
-      if (rok) load_src<NC, S>(ps, r_xy * (uint32_t)sizeof(S), un ? r_xy * 4u : kOOB, r_xy * 4u, p, r);
+    if constexpr (NC == 2) {
+      if (oin) ofl = bld_u8(fl, (uint32_t)o_tile);
+      if (rin) rfl = bld_u8(fl, (uint32_t)r_tile);
+    }
+    RawSrc<NC, S> rwo, rwr;  // reduced storage: converted after all loads
+    if (oin) {
+      if constexpr (sizeof(S) == 4) load_m32(ps, o_xy * 4u, o);
+      else load_raw<NC, S>(ps, o_xy * (uint32_t)sizeof(S), rwo);
+    }
+    if (rin) {
+      if constexpr (sizeof(S) == 4) load_m32(ps, r_xy * 4u, r);
+      else load_raw<NC, S>(ps, r_xy * (uint32_t)sizeof(S), rwr);
+    }
+    ook = oin && (olk & kSolid) == 0u;
+    rok = rin && (rlk & kSolid) == 0u;
+    if (ook) load_extra(ps, ofl ? o_xy * 4u : kOOB, o_xy * 4u, o);
+    if (rok) load_extra(ps, rfl ? r_xy * 4u : kOOB, r_xy * 4u, r);
+    if constexpr (sizeof(S) != 4) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (ook) convert_raw(rwo, p, o);
+      if (rok) convert_raw(rwr, p, r);
     }
   };
 
