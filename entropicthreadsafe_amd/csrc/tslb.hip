@@ -863,34 +863,32 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     ps.fs4 = fs4;
     const rsrc_t fl = rsrc(flags + tiles_plane * (size_t)gz, (uint32_t)tiles_plane);
     const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
-    // Issue order: link masks and curvature flags of both sources, then
-    // both sources' moments (a solid source's too: they are in range and
-    // then unused), then what depends on the flags -- so no load waits on
-    // another before the moments are in flight.
-    const bool oin = zok && oxy, rin = zok && rxy;
-    uint32_t olk = 0, rlk = 0;
-    uint32_t ofl = 1, rfl = 1;  // flags gate the phase-field curvature only
-    if constexpr (MASK) {
-      if (oin) olk = bld_u32(lk, o_xy * 4u);
-      if (rin) rlk = bld_u32(lk, r_xy * 4u);
-    }
-    if constexpr (NC == 2) {
-      if (oin) ofl = bld_u8(fl, (uint32_t)o_tile);
-      if (rin) rfl = bld_u8(fl, (uint32_t)r_tile);
-    }
+    // Per source: link mask, curvature flag, moments, the flag-dependent
+    // loads.  (Issuing both sources' masks and flags ahead of all moments
+    // measured slower: 3.10 against 3.00 ms for 2C FP32.)
     RawSrc<NC, S> rwo, rwr;  // reduced storage: converted after all loads
-    if (oin) {
-      if constexpr (sizeof(S) == 4) load_m32(ps, o_xy * 4u, o);
-      else load_raw<NC, S>(ps, o_xy * (uint32_t)sizeof(S), rwo);
+    ook = zok && oxy;
+    if (ook) {
+      if constexpr (MASK) ook = (bld_u32(lk, o_xy * 4u) & kSolid) == 0u;
+      bool un = NC != 2;  // flags gate the phase-field curvature only
+      if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)o_tile) != 0;
+      if (ook) {
+        if constexpr (sizeof(S) == 4) load_m32(ps, o_xy * 4u, o);
+        else load_raw<NC, S>(ps, o_xy * (uint32_t)sizeof(S), rwo);
+        load_extra(ps, un ? o_xy * 4u : kOOB, o_xy * 4u, o);
+      }
     }
-    if (rin) {
-      if constexpr (sizeof(S) == 4) load_m32(ps, r_xy * 4u, r);
-      else load_raw<NC, S>(ps, r_xy * (uint32_t)sizeof(S), rwr);
+    rok = zok && rxy;
+    if (rok) {
+      if constexpr (MASK) rok = (bld_u32(lk, r_xy * 4u) & kSolid) == 0u;
+      bool un = NC != 2;
+      if constexpr (NC == 2) un = bld_u8(fl, (uint32_t)r_tile) != 0;
+      if (rok) {
+        if constexpr (sizeof(S) == 4) load_m32(ps, r_xy * 4u, r);
+        else load_raw<NC, S>(ps, r_xy * (uint32_t)sizeof(S), rwr);
+        load_extra(ps, un ? r_xy * 4u : kOOB, r_xy * 4u, r);
+      }
     }
-    ook = oin && (olk & kSolid) == 0u;
-    rok = rin && (rlk & kSolid) == 0u;
-    if (ook) load_extra(ps, ofl ? o_xy * 4u : kOOB, o_xy * 4u, o);
-    if (rok) load_extra(ps, rfl ? r_xy * 4u : kOOB, r_xy * 4u, r);
     if constexpr (sizeof(S) != 4) {
       __builtin_amdgcn_sched_barrier(0);
       if (ook) convert_raw(rwo, p, o);
