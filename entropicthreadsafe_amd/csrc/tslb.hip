@@ -39,6 +39,7 @@
 #include <sys/sysinfo.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "tslb.h"
@@ -748,13 +749,15 @@ template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
     const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
     const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, const float *__restrict__ clo,
-    Params p) {
+    int *__restrict__ ilist, uint8_t *__restrict__ flags_next, Params p) {
   constexpr int NQ = NC == 2 ? 26 : 24;
   constexpr int NF = NC >= 2 ? 11 : 10;
   __shared__ float lds[NQ][NSLOT];
   // phase field: phi planes zs-1, zs, zs+1 of the (TX+4)x(TY+4) footprint
   constexpr int NPR = NC == 2 ? 3 : 1;
   __shared__ float phr[NPR][NC == 2 ? NPSLOT : 1];
+  __shared__ uint64_t ilist_wm[NTHR / 64];
+  __shared__ int ilist_base;
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -896,6 +899,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     }
   };
 
+  // Phase field: the next step's list of interface runs, as k_iface_list
+  // builds it, from the planes this block finishes (when ilist is given).
+  uint64_t imask = 0;
+
   // One source plane zs.  aA, aB and aC accumulate the destination planes
   // zs-1 (completed and written here), zs and zs+1; bmA..bmC are their
   // bounce-back masks.  The loop below renames the three accumulators instead
@@ -1016,6 +1023,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 
     // ---- finalize destination plane zs-1
     const int zd = zs - 1;
+    bool hit = false;  // phase field: may the new plane carry curvature here?
     if (active && zd >= z0 && zd < z1) {
       const size_t zb = (size_t)zd * pbS;
       const uint32_t offS = d_xy * (uint32_t)sizeof(S);
@@ -1024,6 +1032,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         for (int q = 0; q < NF; q++)
           bst(rsrc((char *)dst + zb + q * fsS, pbS), offS,
               bld<S>(rsrc((const char *)src + zb + q * fsS, pbS), offS));
+        if constexpr (NC == 2) {
+          float f = to_f(bld<S>(rsrc((const char *)src + zb + TSLB_PHI * fsS, pbS), offS));
+          if constexpr (sizeof(S) != 4) f *= p.isc[TSLB_PHI];
+          hit = f * (1.f - f) > 0.19f;
+        }
       } else {
         const float rho = 1.f + aA.s0;
         // reciprocal with one Newton step (about 0.5 ulp)
@@ -1048,7 +1061,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           if constexpr (sizeof(S) == 4) bst(r, offS, from_f<S>(out[q]));
           else bst(r, offS, from_f<S>(out[q] * p.sc[q]));
         }
+        if constexpr (NC == 2) hit = out[10] * (1.f - out[10]) > 0.19f;
       }
+    }
+    if constexpr (NC == 2) {
+      // the test of k_iface_list on the new state (storage rounding of phi
+      // stays far inside the 0.19 margin), one bit per plane and wave
+      if (zd >= z0 && zd < z1 && __ballot(hit) != 0ull) imask |= 1ull << (zd - z0);
     }
     __syncthreads();
   };
@@ -1071,6 +1090,28 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     body(zs + 1, a1, a2, a0, m1, m2, m0, R1, R2, R0);
     if (zs + 2 > z1) break;
     body(zs + 2, a2, a0, a1, m2, m0, m1, R2, R0, R1);
+  }
+  if constexpr (NC == 2) {
+    if (ilist) {
+      uint64_t *wm = ilist_wm;
+      int *base = &ilist_base;
+      const int tile = tile_y * p.tiles_x + tile_x;
+      if ((tid & 63) == 0) wm[tid >> 6] = imask;
+      if (tid < z1 - z0) flags_next[(z0 + tid) * tiles_plane + tile] = 0;
+      __syncthreads();
+      uint64_t all = 0;
+#pragma unroll
+      for (int w = 0; w < NTHR / 64; w++) all |= wm[w];
+      const uint64_t starts = all & ~(all << 1);
+      if (tid == 0) *base = starts ? atomicAdd(ilist, __popcll(starts)) : 0;
+      __syncthreads();
+      if (tid < z1 - z0 && ((starts >> tid) & 1ull)) {
+        const uint64_t above = ~(all >> tid);
+        const int len = above ? __ffsll((long long)above) - 1 : 64 - tid;
+        ilist[1 + *base + __popcll(starts & ((1ull << tid) - 1ull))] =
+            ((z0 + tid) * (int)tiles_plane + tile) * 64 + len - 1;
+      }
+    }
   }
 }
 
@@ -1959,7 +2000,9 @@ struct tslb_ctx {
   int phi_buf = 0;  // one component: the buffer that holds phi
   float *nrm = nullptr;  // n0, n1, n2, kappa: 4 * fs floats
   uint8_t *flags = nullptr;  // per (tile, plane): kappa (phase field) or gradient nonzero there
-  int *ilist = nullptr;      // phase field: counter, then the tile-planes that may carry curvature
+  int *ilist = nullptr;      // phase field: counter, then the runs of tile-planes that may carry curvature
+  uint8_t *flags_alt = nullptr;  // phase field: flags of the next step (cleared by the fused pass)
+  int list_ok = 0;           // phase field: ilist describes the current state
   int ncu = 256;             // compute units of the device
   uint32_t *links = nullptr;
   int has_mask = 0;
@@ -2085,9 +2128,11 @@ int launch_normals(tslb_ctx *c, int dense = 0) {
       // sparse curvature: list the tile-planes that may carry it, then
       // compute those only
       const S *phi = field_ptr<S>(c->buf[c->cur], c->fs, TSLB_PHI);
-      hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->ilist, 0.f, 1);
-      hipLaunchKernelGGL((k_iface_list<S>), dim3(nblk), dim3(NTHR), 0, c->stream, phi, c->flags, c->ilist,
-                         c->ilist + 1, c->kp);
+      if (!c->list_ok) {  // else the last fused pass listed the runs and cleared the flags
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->ilist, 0.f, 1);
+        hipLaunchKernelGGL((k_iface_list<S>), dim3(nblk), dim3(NTHR), 0, c->stream, phi, c->flags, c->ilist,
+                           c->ilist + 1, c->kp);
+      }
       hipLaunchKernelGGL((k_curv_list<S>), dim3(4 * c->ncu), dim3(NTHR), 0, c->stream, phi, c->nrm + 3 * c->fs,
                          c->flags, c->ilist, c->ilist + 1, c->links, c->has_mask, c->kp);
     }
@@ -2113,6 +2158,11 @@ int launch_fused(tslb_ctx *c) {
   }
   const bool add = p.ncomp != 2 && !p.shift && (p.fx != 0.f || p.fy != 0.f || p.fz != 0.f);
   const bool w4 = (c->variant & 2) != 0;
+  // phase field: the fused pass lists the next step's interface runs unless
+  // a boundary pass still changes phi after it
+  const bool build = p.ncomp == 2 && !c->jet && !c->zbc;
+  int *ilist = build ? c->ilist : nullptr;
+  if (build) hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->ilist, 0.f, 1);
   with_storage(c, [&](auto tag) {
     using S = decltype(tag);
     const S *s = (const S *)c->buf[c->cur];
@@ -2122,10 +2172,10 @@ int launch_fused(tslb_ctx *c) {
     if (c->clo_mode) {                                                                             \
       if constexpr (NC != TSLB_COLOUR)                                                             \
         hipLaunchKernelGGL((k_fused<NC, MASK, ADD, 4, S, true>), dim3(nblk), dim3(NTHR), 0,        \
-                           c->stream, s, d, c->nrm, c->flags, c->links, c->clo, p);                \
+                           c->stream, s, d, c->nrm, c->flags, c->links, c->clo, ilist, c->flags_alt, p); \
     } else {                                                                                       \
       hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, \
-                         d, c->nrm, c->flags, c->links, (const float *)nullptr, p);                \
+                         d, c->nrm, c->flags, c->links, (const float *)nullptr, ilist, c->flags_alt, p); \
     }                                                                                              \
   } while (0)
     // FP32 storage keeps both register budgets selectable for sweeps (capped
@@ -2190,6 +2240,13 @@ int launch_fused(tslb_ctx *c) {
   }
   // one component: phi stays in buf[phi_buf] (check_ctx moves it when needed)
   c->cur = 1 - c->cur;
+  if (build) {
+    // the next curvature pass writes the flags the fused pass just cleared
+    std::swap(c->flags, c->flags_alt);
+    c->list_ok = 1;
+  } else {
+    c->list_ok = 0;
+  }
   c->normals_fresh = 0;
   return TSLB_OK;
 }
@@ -2219,6 +2276,7 @@ int check_ctx_step(tslb_ctx *c) {
 int check_ctx(tslb_ctx *c) {
   int rc = check_ctx_step(c);
   if (rc) return rc;
+  c->list_ok = 0;  // the state may change: the next curvature pass lists afresh
   if (c->prm.ncomp == 1 && c->phi_buf != c->cur) {
     HIPCK(hipMemcpyAsync((char *)c->buf[c->cur] + TSLB_PHI * c->fs * c->es,
                          (char *)c->buf[c->phi_buf] + TSLB_PHI * c->fs * c->es, c->n * c->es,
@@ -2345,6 +2403,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
        (rc = dev_alloc(c, (void **)&c->flags, (size_t)k.tiles_x * k.tiles_y * p->nz))) ||
       (p->ncomp == 2 &&
        (rc = dev_alloc(c, (void **)&c->ilist, ((size_t)k.tiles_x * k.tiles_y * p->nz + 1) * sizeof(int)))) ||
+      (p->ncomp == 2 &&
+       (rc = dev_alloc(c, (void **)&c->flags_alt, (size_t)k.tiles_x * k.tiles_y * p->nz))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
       (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
     std::string msg = g_err;
@@ -2381,6 +2441,7 @@ int tslb_destroy(tslb_ctx *c) {
   (void)hipFree(c->buf[1]);
   (void)hipFree(c->nrm);
   (void)hipFree(c->flags);
+  (void)hipFree(c->flags_alt);
   (void)hipFree(c->ilist);
   (void)hipFree(c->links);
   (void)hipFree(c->part);
