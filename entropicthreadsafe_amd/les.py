@@ -115,7 +115,7 @@ def _timed_steps(lat, steps):
 
 
 def run(n=512, r=4, k0=4.0, urms=0.04, spinup=1500, snaps=4, snap_every=250, horizon=2000,
-        train_steps=3000, batch=16384, lr=2e-3, seed=1, drops=True, log=print):
+        train_steps=9000, batch=16384, lr=2e-3, seed=1, drops=True, log=print):
     """The experiment; returns a dict of its results.  drops=False runs the
     same two-component lattice without droplets (phi = 0): single-phase
     turbulence, a check of the coarse-grid scaling."""
@@ -172,6 +172,11 @@ def run(n=512, r=4, k0=4.0, urms=0.04, spinup=1500, snaps=4, snap_every=250, hor
             with _coarse_lattice(nc, kw, r, rb, vel, phi) as lat:
                 if name == "closure":
                     lat.set_closure(model)
+                    clo = lat.download_closure()   # fields of the t0 state
+                    res["closure_nu_t_mean"] = float(clo[0].mean())
+                    res["closure_nu_t_max"] = float(clo[0].max())
+                    res["closure_dsigma_rms"] = float(np.sqrt((clo[1:].astype(np.float64) ** 2).mean()))
+                    res["nu_coarse"] = (0.5 + (kw["tau"] - 0.5) / r - 0.5) / 3.0
                 dt = _timed_steps(lat, horizon // r)
                 m = lat.download()
                 if name == "closure":
@@ -214,7 +219,7 @@ if __name__ == "__main__":
     ap.add_argument("--factor", type=int, default=4)
     ap.add_argument("--spinup", type=int, default=1500)
     ap.add_argument("--horizon", type=int, default=2000)
-    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-steps", type=int, default=9000)
     ap.add_argument("--no-drops", action="store_true")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
