@@ -377,6 +377,58 @@ __device__ __forceinline__ void phi_normal(const Get &get, float &n0, float &n1,
   n2 = gz * inv;
 }
 
+// The same stencil, separated.  Its weights 2/27, 1/54 and 1/216 (faces,
+// edges, corners) are W(dx) W(dy) W(dz) / 216 with W(0) = 4, W(+-1) = 1, so
+// with the in-plane sums of plane z
+//   Px(z) = sum_dy W(dy) (phi(x+1, y+dy, z) - phi(x-1, y+dy, z)),  Py alike,
+//   Ps(z) = sum_dx,dy W(dx) W(dy) phi(x+dx, y+dy, z)
+// the gradient is (3/216) (Px(z-1) + 4 Px(z) + Px(z+1), Py(..), Ps(z+1) - Ps(z-1)).
+// A march along z forms Px, Py, Ps once per plane from the 3x3 in-plane
+// neighbourhood (centre c of a phi plane of row pitch PW) and carries the
+// z-combination in registers.
+struct PlaneGrad {
+  float x, y, s;
+};
+__device__ __forceinline__ PlaneGrad plane_grad(const float *pl, int c) {
+  const float a = pl[c - PW - 1], b = pl[c - PW], d = pl[c - PW + 1];
+  const float e = pl[c - 1], f = pl[c], g = pl[c + 1];
+  const float h = pl[c + PW - 1], i = pl[c + PW], j = pl[c + PW + 1];
+  PlaneGrad r;
+  r.x = (d - a) + fmaf(4.f, g - e, j - h);
+  r.y = (h - a) + fmaf(4.f, i - b, j - d);
+  const float cm = fmaf(4.f, e, a + h), c0 = fmaf(4.f, f, b + i), cp = fmaf(4.f, g, d + j);
+  r.s = fmaf(4.f, c0, cm + cp);
+  return r;
+}
+// running z-combination of one column: rx = Px(z-1) + 4 Px(z), lx = Px(z),
+// sm = Ps(z-1), sz = Ps(z) (and y alike)
+struct GradRun {
+  float rx, lx, ry, ly, sm, sz;
+};
+__device__ __forceinline__ void grad_run_init(GradRun &g, const PlaneGrad &a, const PlaneGrad &b) {
+  g.rx = fmaf(4.f, b.x, a.x);
+  g.lx = b.x;
+  g.ry = fmaf(4.f, b.y, a.y);
+  g.ly = b.y;
+  g.sm = a.s;
+  g.sz = b.s;
+}
+// the unit normal of plane z from the sums of plane z+1, and the advance to z+1;
+// the gradient is kept as 72 grad(phi), so the 1e-9 floor becomes 7.2e-8
+__device__ __forceinline__ void grad_run_step(GradRun &g, const PlaneGrad &n, float *nn) {
+  const float gx = g.rx + n.x, gy = g.ry + n.y, gz = n.s - g.sm;
+  const float inv = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(gx, gx, fmaf(gy, gy, gz * gz))) + 7.2e-8f);
+  nn[0] = gx * inv;
+  nn[1] = gy * inv;
+  nn[2] = gz * inv;
+  g.rx = fmaf(4.f, n.x, g.lx);
+  g.lx = n.x;
+  g.ry = fmaf(4.f, n.y, g.ly);
+  g.ly = n.y;
+  g.sm = g.sz;
+  g.sz = n.s;
+}
+
 // phi stencil coordinate along one axis: wrapped across a periodic face,
 // the nearest interior node across a wall
 __device__ __forceinline__ int clampw(int g, int n, int per) {
@@ -820,15 +872,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       if (ptwo) d[tid + NTHR] = to_f(pr1) * p.isc[TSLB_PHI];
     }
   };
-  // normal of the source at footprint centre c and its phi (planes pm, p0, pp)
-  auto src_normal = [&](Src<NC, CLO> &sc, int c, const float *pm, const float *p0, const float *pp) {
-    auto get = [&](int dx, int dy, int dz) -> float {
-      const float *pl = dz < 0 ? pm : (dz > 0 ? pp : p0);
-      return pl[c + dy * PW + dx];
-    };
-    phi_normal(get, sc.nn[0], sc.nn[1], sc.nn[2]);
-    sc.m[NF - 1] = p0[c];
-  };
+  // Phase field: the normals of both sources come from the separated
+  // gradient stencil, marched along z with the source planes (footprint
+  // centres co_c and cr_c of the phi planes).
+  const int co_c = (ty + 2) * PW + tx + 2, cr_c = (rsy + 1) * PW + rsx + 1;
+  GradRun gro, grr;
 
   // ghost-wall bounce-back bits of this column
   uint32_t wall_xy = 0;
@@ -920,9 +968,19 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
     float gax = 0.f, gay = 0.f, gb = 0.f;  // phase field: own A_x, A_y, phi/8
     float oc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // colour: cA, bc, g
+    float nno[3], nnr[3];
+    if constexpr (NC == 2) {
+      grad_run_step(gro, plane_grad(pp, co_c), nno);
+      if (tid < NRING) grad_run_step(grr, plane_grad(pp, cr_c), nnr);
+    }
     // ---- own source slot
     if (co_ok) {
-      if constexpr (NC == 2) src_normal(co, (ty + 2) * PW + tx + 2, pm, p0, pp);
+      if constexpr (NC == 2) {
+        co.nn[0] = nno[0];
+        co.nn[1] = nno[1];
+        co.nn[2] = nno[2];
+        co.m[NF - 1] = p0[co_c];
+      }
       PopSink sk{lds, (ty + 1) * SW + (tx + 1), 0.f, 0.f, 0.f};
       eval_src<NC, ADD>(co, p, sk, gq);
       h0 = sk.h0;
@@ -941,7 +999,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     }
     // ---- halo ring slot
     if (cr_ok) {
-      if constexpr (NC == 2) src_normal(cr, (rsy + 1) * PW + rsx + 1, pm, p0, pp);
+      if constexpr (NC == 2) {
+        cr.nn[0] = nnr[0];
+        cr.nn[1] = nnr[1];
+        cr.nn[2] = nnr[2];
+        cr.m[NF - 1] = p0[cr_c];
+      }
       PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
       eval_src<NC, ADD>(cr, p, sk, gq);
       if constexpr (NC == 3)
@@ -1083,6 +1146,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       phi_commit(phr[k < NPR ? k : 0]);
     }
     __syncthreads();
+    grad_run_init(gro, plane_grad(R0, co_c), plane_grad(R1, co_c));
+    if (tid < NRING) grad_run_init(grr, plane_grad(R0, cr_c), plane_grad(R1, cr_c));
   }
   for (int zs = z0 - 1; zs <= z1; zs += 3) {
     body(zs, a0, a1, a2, m0, m1, m2, R0, R1, R2);
