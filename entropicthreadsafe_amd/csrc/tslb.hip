@@ -797,6 +797,20 @@ __device__ __forceinline__ float bb_sum_layer(const float (*lds)[NSLOT], int tx,
   return s;
 }
 
+// Runs of interface tile-planes are cut at every kRunMax-th plane of a
+// z-chunk, so that no block of the curvature pass marches more than kRunMax
+// planes: the long runs of a chunk otherwise set the length of the launch.
+constexpr int kRunMax = 16;
+constexpr uint64_t kRunCut = 0x0001000100010001ull;  // planes 0, 16, 32, 48 of a chunk
+static_assert(kRunMax == 16, "kRunCut marks every 16th plane");
+// starts of the runs in a 64-plane hit mask, and the length of the run at k
+__device__ __forceinline__ uint64_t run_starts(uint64_t all) { return (all & ~(all << 1)) | (all & kRunCut); }
+__device__ __forceinline__ int run_len(uint64_t all, int k) {
+  const uint64_t above = ~(all >> k);  // first clear bit at or above k ends the run
+  const int len = above ? __ffsll((long long)above) - 1 : 64 - k;
+  return min(len, kRunMax - (k & (kRunMax - 1)));
+}
+
 template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
     const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
@@ -1167,12 +1181,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       uint64_t all = 0;
 #pragma unroll
       for (int w = 0; w < NTHR / 64; w++) all |= wm[w];
-      const uint64_t starts = all & ~(all << 1);
+      const uint64_t starts = run_starts(all);
       if (tid == 0) *base = starts ? atomicAdd(ilist, __popcll(starts)) : 0;
       __syncthreads();
       if (tid < z1 - z0 && ((starts >> tid) & 1ull)) {
-        const uint64_t above = ~(all >> tid);
-        const int len = above ? __ffsll((long long)above) - 1 : 64 - tid;
+        const int len = run_len(all, tid);
         ilist[1 + *base + __popcll(starts & ((1ull << tid) - 1ull))] =
             ((z0 + tid) * (int)tiles_plane + tile) * 64 + len - 1;
       }
@@ -1465,8 +1478,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
 // (at most 64 planes) with the phi plane four ahead in flight; each wave
 // keeps a 64-bit mask of its planes with a hit (a ballot, no barrier), the
 // block ORs the eight masks once at the end and appends its runs with one
-// atomic on the counter (a vector atomic).  Entry: 64 (z_first * tiles per
-// plane + tile) + run length - 1.
+// atomic on the counter (a vector atomic); runs are cut at every 16th plane
+// of the chunk.  Entry: 64 (z_first * tiles per plane + tile) + run length - 1.
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, uint8_t *__restrict__ flags,
                                                      int *__restrict__ cnt, int *__restrict__ list, Params p) {
@@ -1516,14 +1529,13 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
   uint64_t all = 0;
 #pragma unroll
   for (int w = 0; w < NTHR / 64; w++) all |= wmask[w];
-  // run starts: set bits whose lower neighbour is clear
-  const uint64_t starts = all & ~(all << 1);
+  // run starts: set bits whose lower neighbour is clear, and the cuts
+  const uint64_t starts = run_starts(all);
   if (tid == 0) base = starts ? atomicAdd(cnt, __popcll(starts)) : 0;
   __syncthreads();
   // thread k appends the run starting at plane z0 + k, at its rank among the starts
   if (tid < z1 - z0 && ((starts >> tid) & 1ull)) {
-    const uint64_t above = ~(all >> tid);  // first clear bit at or above k ends the run
-    const int len = above ? __ffsll((long long)above) - 1 : 64 - tid;
+    const int len = run_len(all, tid);
     list[base + __popcll(starts & ((1ull << tid) - 1ull))] = ((z0 + tid) * tiles_plane + tile) * 64 + len - 1;
   }
 }
@@ -1568,13 +1580,14 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
     auto r5 = [zs](int z) { return (z - zs + 10) % 5; };
     auto r3 = [zs](int z) { return (z - zs + 3) % 3; };
     S qa{}, qb{};
-    auto issue = [&](int z) {
+    auto issue_to = [&](int z, S &ra, S &rb) {
       const rsrc_t r = rsrc(phi + plane * (size_t)clampw(z, p.nz, p.pz), pbS);
-      qa = bld<S>(r, o0);
-      if (two) qb = bld<S>(r, o1);
+      ra = bld<S>(r, o0);
+      if (two) rb = bld<S>(r, o1);
     };
-    auto commit = [&](int z) {
-      float a = to_f(qa), c = two ? to_f(qb) : 0.f;
+    auto issue = [&](int z) { issue_to(z, qa, qb); };
+    auto commit_from = [&](int z, S ra, S rb) {
+      float a = to_f(ra), c = two ? to_f(rb) : 0.f;
       if constexpr (sizeof(S) != 4) {
         a *= p.isc[TSLB_PHI];
         c *= p.isc[TSLB_PHI];
@@ -1582,6 +1595,7 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       ph[r5(z)][tid] = a;
       if (two) ph[r5(z)][tid + NTHR] = c;
     };
+    auto commit = [&](int z) { commit_from(z, qa, qb); };
     // n of plane zz from phi planes zz-1..zz+1
     auto normals = [&](int zz) {
       int gz = 0;
@@ -1613,12 +1627,16 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       one(tid);
       if (ntwo) one(tid + NTHR);
     };
-    // prologue: phi planes zs-2..zs+2, n of planes zs-1 and zs
-    for (int k = 0; k < 5; k++) {
-      issue(zs - 2 + k);
-      commit(zs - 2 + k);
+    // prologue: phi planes zs-2..zs+2 (all five loads in flight at once), n
+    // of planes zs-1 and zs
+    {
+      S pa[5]{}, pb[5]{};
+#pragma unroll
+      for (int k = 0; k < 5; k++) issue_to(zs - 2 + k, pa[k], pb[k]);
+      if (len > 1) issue(zs + 3);
+#pragma unroll
+      for (int k = 0; k < 5; k++) commit_from(zs - 2 + k, pa[k], pb[k]);
     }
-    if (len > 1) issue(zs + 3);
     __syncthreads();
     normals(zs - 1);
     normals(zs);
