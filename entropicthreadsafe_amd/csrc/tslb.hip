@@ -800,9 +800,9 @@ __device__ __forceinline__ float bb_sum_layer(const float (*lds)[NSLOT], int tx,
 // Runs of interface tile-planes are cut at every kRunMax-th plane of a
 // z-chunk, so that no block of the curvature pass marches more than kRunMax
 // planes: the long runs of a chunk otherwise set the length of the launch.
-constexpr int kRunMax = 16;
-constexpr uint64_t kRunCut = 0x0001000100010001ull;  // planes 0, 16, 32, 48 of a chunk
-static_assert(kRunMax == 16, "kRunCut marks every 16th plane");
+constexpr int kRunMax = 8;
+constexpr uint64_t kRunCut = 0x0101010101010101ull;  // every 8th plane of a chunk
+static_assert(kRunMax == 8, "kRunCut marks every 8th plane");
 // starts of the runs in a 64-plane hit mask, and the length of the run at k
 __device__ __forceinline__ uint64_t run_starts(uint64_t all) { return (all & ~(all << 1)) | (all & kRunCut); }
 __device__ __forceinline__ int run_len(uint64_t all, int k) {
@@ -1478,7 +1478,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
 // (at most 64 planes) with the phi plane four ahead in flight; each wave
 // keeps a 64-bit mask of its planes with a hit (a ballot, no barrier), the
 // block ORs the eight masks once at the end and appends its runs with one
-// atomic on the counter (a vector atomic); runs are cut at every 16th plane
+// atomic on the counter (a vector atomic); runs are cut at every 8th plane
 // of the chunk.  Entry: 64 (z_first * tiles per plane + tile) + run length - 1.
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, uint8_t *__restrict__ flags,
