@@ -366,45 +366,43 @@ __device__ __forceinline__ float grad_axis(const Get &get) {
   return 3.f * (wgt(1) * (fp - fm) + wgt(7) * (ep - em) + wgt(19) * (cp - cmn));
 }
 
-// unit normal n = grad(phi)/(|grad(phi)| + 1e-9) (recursive_2c_TSLB3D.f90:
-// 555-571); get(dx, dy, dz) returns phi of the neighbour
-template <typename Get>
-__device__ __forceinline__ void phi_normal(const Get &get, float &n0, float &n1, float &n2) {
-  const float gx = grad_axis<0>(get), gy = grad_axis<1>(get), gz = grad_axis<2>(get);
-  const float inv = 1.f / (sqrtf(gx * gx + gy * gy + gz * gz) + 1e-9f);
-  n0 = gx * inv;
-  n1 = gy * inv;
-  n2 = gz * inv;
-}
-
-// The same stencil, separated.  Its weights 2/27, 1/54 and 1/216 (faces,
-// edges, corners) are W(dx) W(dy) W(dz) / 216 with W(0) = 4, W(+-1) = 1, so
-// with the in-plane sums of plane z
+// Unit normal n = grad(phi)/(|grad(phi)| + 1e-9) (recursive_2c_TSLB3D.f90:
+// 555-571) from the same stencil, separated.  Its weights 2/27, 1/54 and
+// 1/216 (faces, edges, corners) are W(dx) W(dy) W(dz) / 216 with W(0) = 4,
+// W(+-1) = 1, so with the in-plane sums of plane z (centre value pc(z))
 //   Px(z) = sum_dy W(dy) (phi(x+1, y+dy, z) - phi(x-1, y+dy, z)),  Py alike,
-//   Ps(z) = sum_dx,dy W(dx) W(dy) phi(x+dx, y+dy, z)
-// the gradient is (3/216) (Px(z-1) + 4 Px(z) + Px(z+1), Py(..), Ps(z+1) - Ps(z-1)).
-// A march along z forms Px, Py, Ps once per plane from the 3x3 in-plane
-// neighbourhood (centre c of a phi plane of row pitch PW) and carries the
-// z-combination in registers.
+//   Ps(z) = sum_dx,dy W(dx) W(dy) (phi(x+dx, y+dy, z) - pc(z))
+// the gradient is (3/216) (Px(z-1) + 4 Px(z) + Px(z+1), Py(..),
+// Ps(z+1) - Ps(z-1) + 36 (pc(z+1) - pc(z-1))).  Every term is a difference
+// of nearby phi before any weighted sum, so the sums carry no cancellation:
+// in a bulk phase (|grad phi| ~ 1e-4, phi ~ 1) the FP32 normal stays within
+// ~1e-6 of the FP64 one, where summing each side of the stencil first (as
+// the reference does) loses ~1e-4.  A march along z forms Px, Py, Ps once
+// per plane from the 3x3 in-plane neighbourhood (centre c of phi planes of
+// row pitch PITCH) and carries the z-combination in registers.
 struct PlaneGrad {
-  float x, y, s;
+  float x, y, s, c;
 };
+template <int PITCH = PW>
 __device__ __forceinline__ PlaneGrad plane_grad(const float *pl, int c) {
-  const float a = pl[c - PW - 1], b = pl[c - PW], d = pl[c - PW + 1];
+  const float a = pl[c - PITCH - 1], b = pl[c - PITCH], d = pl[c - PITCH + 1];
   const float e = pl[c - 1], f = pl[c], g = pl[c + 1];
-  const float h = pl[c + PW - 1], i = pl[c + PW], j = pl[c + PW + 1];
+  const float h = pl[c + PITCH - 1], i = pl[c + PITCH], j = pl[c + PITCH + 1];
   PlaneGrad r;
   r.x = (d - a) + fmaf(4.f, g - e, j - h);
   r.y = (h - a) + fmaf(4.f, i - b, j - d);
-  const float cm = fmaf(4.f, e, a + h), c0 = fmaf(4.f, f, b + i), cp = fmaf(4.f, g, d + j);
+  const float cm = fmaf(4.f, e - f, (a - f) + (h - f)), c0 = (b - f) + (i - f);
+  const float cp = fmaf(4.f, g - f, (d - f) + (j - f));
   r.s = fmaf(4.f, c0, cm + cp);
+  r.c = f;
   return r;
 }
-// running z-combination of one column: rx = Px(z-1) + 4 Px(z), lx = Px(z),
-// sm = Ps(z-1), sz = Ps(z) (and y alike)
+// running z-combination of one column: rx = Px(z-1) + 4 Px(z), lx = Px(z)
+// (and y alike), sm = Ps(z-1), sz = Ps(z)
 struct GradRun {
   float rx, lx, ry, ly, sm, sz;
 };
+// a, b: the sums of planes z-1 and z
 __device__ __forceinline__ void grad_run_init(GradRun &g, const PlaneGrad &a, const PlaneGrad &b) {
   g.rx = fmaf(4.f, b.x, a.x);
   g.lx = b.x;
@@ -413,10 +411,11 @@ __device__ __forceinline__ void grad_run_init(GradRun &g, const PlaneGrad &a, co
   g.sm = a.s;
   g.sz = b.s;
 }
-// the unit normal of plane z from the sums of plane z+1, and the advance to z+1;
-// the gradient is kept as 72 grad(phi), so the 1e-9 floor becomes 7.2e-8
-__device__ __forceinline__ void grad_run_step(GradRun &g, const PlaneGrad &n, float *nn) {
-  const float gx = g.rx + n.x, gy = g.ry + n.y, gz = n.s - g.sm;
+// the unit normal of plane z from the sums of plane z+1 and the centre value
+// of plane z-1 (pcm), and the advance to z+1; the gradient is kept as
+// 72 grad(phi), so the 1e-9 floor becomes 7.2e-8
+__device__ __forceinline__ void grad_run_step(GradRun &g, const PlaneGrad &n, float pcm, float *nn) {
+  const float gx = g.rx + n.x, gy = g.ry + n.y, gz = fmaf(36.f, n.c - pcm, n.s - g.sm);
   const float inv = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(gx, gx, fmaf(gy, gy, gz * gz))) + 7.2e-8f);
   nn[0] = gx * inv;
   nn[1] = gy * inv;
@@ -427,6 +426,19 @@ __device__ __forceinline__ void grad_run_step(GradRun &g, const PlaneGrad &n, fl
   g.ly = n.y;
   g.sm = g.sz;
   g.sz = n.s;
+}
+// the unit normal at centre c from phi planes z-1, z, z+1: the operations of
+// the march in the same order, so every pass gets the same bits
+template <int PITCH = PW>
+__device__ __forceinline__ void sep_normal(const float *P0, const float *P1, const float *P2, int c,
+                                           float &n0, float &n1, float &n2) {
+  GradRun g;
+  grad_run_init(g, plane_grad<PITCH>(P0, c), plane_grad<PITCH>(P1, c));
+  float nn[3];
+  grad_run_step(g, plane_grad<PITCH>(P2, c), P0[c], nn);
+  n0 = nn[0];
+  n1 = nn[1];
+  n2 = nn[2];
 }
 
 // phi stencil coordinate along one axis: wrapped across a periodic face,
@@ -984,8 +996,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     float oc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // colour: cA, bc, g
     float nno[3], nnr[3];
     if constexpr (NC == 2) {
-      grad_run_step(gro, plane_grad(pp, co_c), nno);
-      if (tid < NRING) grad_run_step(grr, plane_grad(pp, cr_c), nnr);
+      grad_run_step(gro, plane_grad(pp, co_c), pm[co_c], nno);
+      if (tid < NRING) grad_run_step(grr, plane_grad(pp, cr_c), pm[cr_c], nnr);
     }
     // ---- own source slot
     if (co_ok) {
@@ -1347,7 +1359,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
             bb = grad_axis<1>(get);
             cc = grad_axis<2>(get);
           } else {
-            phi_normal(get, a, bb, cc);
+            sep_normal(P0, P1, P2, c, a, bb, cc);
           }
         }
       }
@@ -1596,36 +1608,32 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       if (two) ph[r5(z)][tid + NTHR] = c;
     };
     auto commit = [&](int z) { commit_from(z, qa, qb); };
-    // n of plane zz from phi planes zz-1..zz+1
+    // n of plane zz from the in-plane sums of phi plane zz+1, the centre of
+    // plane zz-1 and the z-march
+    // of each normal slot (g0: slot tid, g1: slot tid + NTHR)
+    const int c0 = (tid / SW + 1) * PW + tid % SW + 1;
+    const int c1 = ntwo ? ((tid + NTHR) / SW + 1) * PW + (tid + NTHR) % SW + 1 : c0;
+    GradRun g0, g1;
     auto normals = [&](int zz) {
       int gz = 0;
       const bool zin = src_coord(zz, p.nz, p.pz, gz);
       const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
-      const float *P0 = ph[r5(zz - 1)], *P1 = ph[r5(zz)], *P2 = ph[r5(zz + 1)];
+      const float *P0 = ph[r5(zz - 1)], *P2 = ph[r5(zz + 1)];
       float(*dn)[NSLOT] = nl[r3(zz)];
-      auto one = [&](int sl) {
+      auto one = [&](int sl, int c, GradRun &g) {
+        float nn[3];
+        grad_run_step(g, plane_grad(P2, c), P0[c], nn);
         const int sx = sl % SW, sy = sl / SW;
         int gx = 0, gy = 0;
         const bool okxy = src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
-        float a = 0.f, bb = 0.f, cc = 0.f;
-        if (zin && okxy) {
-          bool ok = true;
-          if (has_mask) ok = (bld_u32(lk, (uint32_t)(gx + p.nx * gy) * 4u) & kSolid) == 0u;
-          if (ok) {
-            const int c = (sy + 1) * PW + (sx + 1);
-            auto get = [&](int dx, int dy, int dz) -> float {
-              const float *pl = dz < 0 ? P0 : (dz > 0 ? P2 : P1);
-              return pl[c + dy * PW + dx];
-            };
-            phi_normal(get, a, bb, cc);
-          }
-        }
-        dn[0][sl] = a;
-        dn[1][sl] = bb;
-        dn[2][sl] = cc;
+        bool ok = zin && okxy;
+        if (ok && has_mask) ok = (bld_u32(lk, (uint32_t)(gx + p.nx * gy) * 4u) & kSolid) == 0u;
+        dn[0][sl] = ok ? nn[0] : 0.f;
+        dn[1][sl] = ok ? nn[1] : 0.f;
+        dn[2][sl] = ok ? nn[2] : 0.f;
       };
-      one(tid);
-      if (ntwo) one(tid + NTHR);
+      one(tid, c0, g0);
+      if (ntwo) one(tid + NTHR, c1, g1);
     };
     // prologue: phi planes zs-2..zs+2 (all five loads in flight at once), n
     // of planes zs-1 and zs
@@ -1638,6 +1646,8 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       for (int k = 0; k < 5; k++) commit_from(zs - 2 + k, pa[k], pb[k]);
     }
     __syncthreads();
+    grad_run_init(g0, plane_grad(ph[r5(zs - 2)], c0), plane_grad(ph[r5(zs - 1)], c0));
+    if (ntwo) grad_run_init(g1, plane_grad(ph[r5(zs - 2)], c1), plane_grad(ph[r5(zs - 1)], c1));
     normals(zs - 1);
     normals(zs);
     for (int z = zs; z < zs + len; z++) {
@@ -1839,11 +1849,17 @@ __global__ __launch_bounds__(256) void k_jet_bc(const S *__restrict__ src, S *__
   for (int q = 0; q < (NC == 2 ? 11 : 10); q++) sy.m[q] = at(q, y);
   if constexpr (NC == 2) {
     // n from phi as the fused pass computes it; planes zy +- 1 are in range
-    auto get = [&](int dx, int dy, int dz) -> float {
-      const int gi = clampw(i + dx, p.nx, p.px), gj = clampw(j + dy, p.ny, p.py);
-      return at(TSLB_PHI, gi + p.nx * gj + plane * (zy + dz));
-    };
-    phi_normal(get, sy.nn[0], sy.nn[1], sy.nn[2]);
+    float ph3[3][9];
+#pragma unroll
+    for (int dz = -1; dz <= 1; dz++)
+#pragma unroll
+      for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+          const int gi = clampw(i + dx, p.nx, p.px), gj = clampw(j + dy, p.ny, p.py);
+          ph3[dz + 1][(dy + 1) * 3 + dx + 1] = at(TSLB_PHI, gi + p.nx * gj + plane * (zy + dz));
+        }
+    sep_normal<3>(ph3[0], ph3[1], ph3[2], 4, sy.nn[0], sy.nn[1], sy.nn[2]);
     const int tile = (j / TY) * p.tiles_x + i / TX;
     const bool un = flags[(size_t)zy * p.tiles_x * p.tiles_y + tile] != 0;
     sy.kap = un ? nrm[3 * (size_t)p.fs + y] : 0.f;

@@ -231,7 +231,13 @@ def test_fused_interface_list_equals_fresh_list(storage):
 
 
 def test_normals_match_oracle():
-    # K2 on the GPU's own phi: the oracle's step starts with K2 of its state
+    """K2 on the GPU's own phi (the oracle's step starts with K2 of its state).
+
+    The GPU forms the stencil from differences of neighbouring phi before any
+    weighted sum; the reference sums each side first, which in a bulk phase
+    (|grad phi| ~ 1e-4, phi ~ 1) loses ~1e-4 of the unit normal to rounding
+    in its own FP32 arithmetic.  The GPU is held to the FP64 oracle at every
+    node (an FP32 emulation of its order stays within 5e-7)."""
     case = configs.droplet(n=24, radius=6.0)
     f = case.host_fields()
     with Lattice(24, 24, 24, **case.lattice) as lat:
@@ -241,11 +247,21 @@ def test_normals_match_oracle():
         lat.step(3)
         n_gpu = lat.download_normals()
         m = lat.download()
-    o = po.Oracle(24, 24, 24, **_oracle_kwargs(case.lattice), dtype=np.float32)
-    o.set_moments(m)
-    o.step(1)
-    n_ref = o.normals()
-    assert np.abs(n_gpu - n_ref).max() < 1e-5
+    ref = {}
+    for dt in (np.float32, np.float64):
+        o = po.Oracle(24, 24, 24, **_oracle_kwargs(case.lattice), dtype=dt)
+        o.set_moments(m.astype(dt))
+        o.step(1)
+        ref[dt] = o.normals().astype(np.float64)
+        o.close()
+    err = np.abs(n_gpu - ref[np.float64])
+    e_ref = np.abs(ref[np.float32] - ref[np.float64])
+    phi = m[10].astype(np.float64)
+    iface = phi * (1 - phi) > 0.05
+    print("normals: GPU vs FP64 max", err.max(), "at interface", err[:, iface].max(),
+          "| FP32 oracle vs FP64 max", e_ref.max())
+    assert iface.sum() > 500
+    assert err.max() < 1e-5
 
 
 def test_device_noise_matches_host_generator():
