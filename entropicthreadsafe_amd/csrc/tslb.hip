@@ -483,14 +483,20 @@ template <>
 __device__ __forceinline__ bf16_t bld<bf16_t>(rsrc_t r, uint32_t off) {
   return bf16_t{__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)};
 }
+// CP: cache-policy bits of the store; 2 (nt) marks it non-temporal: the
+// fused pass writes the next state once and reads none of it back, and its
+// stores then leave the L2 to the halo rows the neighbouring tiles re-read
+template <int CP = 0>
 __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, CP);
 }
+template <int CP = 0>
 __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, _Float16 v) {
-  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)off, 0, CP);
 }
+template <int CP = 0>
 __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, bf16_t v) {
-  __builtin_amdgcn_raw_buffer_store_b16(v.b, r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b16(v.b, r, (int)off, 0, CP);
 }
 __device__ __forceinline__ uint32_t bld_u32(rsrc_t r, uint32_t off) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
@@ -1119,7 +1125,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       if (MASK && (bmA & kSolid)) {
 #pragma unroll
         for (int q = 0; q < NF; q++)
-          bst(rsrc((char *)dst + zb + q * fsS, pbS), offS,
+          bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS,
               bld<S>(rsrc((const char *)src + zb + q * fsS, pbS), offS));
         if constexpr (NC == 2) {
           float f = to_f(bld<S>(rsrc((const char *)src + zb + TSLB_PHI * fsS, pbS), offS));
@@ -1147,8 +1153,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 #pragma unroll
         for (int q = 0; q < NF; q++) {
           const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
-          if constexpr (sizeof(S) == 4) bst(r, offS, from_f<S>(out[q]));
-          else bst(r, offS, from_f<S>(out[q] * p.sc[q]));
+          if constexpr (sizeof(S) == 4) bst<2>(r, offS, from_f<S>(out[q]));
+          else bst<2>(r, offS, from_f<S>(out[q] * p.sc[q]));
         }
         if constexpr (NC == 2) hit = out[10] * (1.f - out[10]) > 0.19f;
       }
