@@ -262,18 +262,18 @@ __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, 
     a1y += p.fy;
     a1z += p.fz;
   }
-  const float rVxx = rVx * Vx, rVyy = rVy * Vy, rVzz = rVz * Vz;
   const float rVxy = rVx * Vy, rVxz = rVx * Vz, rVyz = rVy * Vz;
-  const float a2xx = rVxx + Bxx, a2yy = rVyy + Byy, a2zz = rVzz + Bzz;
+  const float a2xx = fmaf(rVx, Vx, Bxx), a2yy = fmaf(rVy, Vy, Byy), a2zz = fmaf(rVz, Vz, Bzz);
   const float a2xy = rVxy + Bxy, a2xz = rVxz + Bxz, a2yz = rVyz + Byz;
-  const float a3xxy = fmaf(rVxx, Vy, fmaf(uy, Bxx, 2.f * ux * Bxy));
-  const float a3xxz = fmaf(rVxx, Vz, fmaf(uz, Bxx, 2.f * ux * Bxz));
-  const float a3xyy = fmaf(rVyy, Vx, fmaf(ux, Byy, 2.f * uy * Bxy));
-  const float a3yyz = fmaf(rVyy, Vz, fmaf(uz, Byy, 2.f * uy * Byz));
-  const float a3xzz = fmaf(rVzz, Vx, fmaf(ux, Bzz, 2.f * uz * Bxz));
-  const float a3yzz = fmaf(rVzz, Vy, fmaf(uy, Bzz, 2.f * uz * Byz));
+  // rho Va Va Vb taken as (rho Va Vb) Va
+  const float a3xxy = fmaf(rVxy, Vx, fmaf(uy, Bxx, 2.f * ux * Bxy));
+  const float a3xxz = fmaf(rVxz, Vx, fmaf(uz, Bxx, 2.f * ux * Bxz));
+  const float a3xyy = fmaf(rVxy, Vy, fmaf(ux, Byy, 2.f * uy * Bxy));
+  const float a3yyz = fmaf(rVyz, Vy, fmaf(uz, Byy, 2.f * uy * Byz));
+  const float a3xzz = fmaf(rVxz, Vz, fmaf(ux, Bzz, 2.f * uz * Bxz));
+  const float a3yzz = fmaf(rVyz, Vz, fmaf(uy, Bzz, 2.f * uz * Byz));
   const float a3xyz = fmaf(rVxy, Vz, fmaf(ux, Byz, fmaf(uy, Bxz, uz * Bxy)));
-  constexpr float c3 = 1.f / 3.f, c9 = 1.f / 9.f, c27 = 1.f / 27.f;
+  constexpr float c3 = 1.f / 3.f, c9 = 1.f / 9.f;
   // M[a][b][c], scaled by 2^-(#nonzero)
   float M[3][3][3];
   M[0][0][0] = drho;
@@ -293,16 +293,22 @@ __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, 
   M[1][0][2] = 0.25f * fmaf(c3, a1x, a3xzz);
   M[0][1][2] = 0.25f * fmaf(c3, a1y, a3yzz);
   M[1][1][1] = 0.125f * a3xyz;
-  M[2][2][0] = 0.25f * fmaf(c9, drho, c3 * (a2xx + a2yy));
-  M[2][0][2] = 0.25f * fmaf(c9, drho, c3 * (a2xx + a2zz));
-  M[0][2][2] = 0.25f * fmaf(c9, drho, c3 * (a2yy + a2zz));
+  // products of per-axis factors from the scaled entries above:
+  //   M220 = 1/4 (c9 drho + c3 (a2xx + a2yy)) = c3/2 (M200 + M020) - c9/4 drho,
+  //   M222 = 1/8 (c27 drho + c9 (a2xx + a2yy + a2zz)) = c9/4 (M200 + M020 + M002) - drho/108,
+  //   M221 = 1/8 (c9 a1z + c3 (a3xxz + a3yyz)) = c3/2 (M201 + M021) - c9/4 M001
+  const float e9 = (0.25f * c9) * drho;
+  const float s200_020 = M[2][0][0] + M[0][2][0];
+  M[2][2][0] = fmaf(0.5f * c3, s200_020, -e9);
+  M[2][0][2] = fmaf(0.5f * c3, M[2][0][0] + M[0][0][2], -e9);
+  M[0][2][2] = fmaf(0.5f * c3, M[0][2][0] + M[0][0][2], -e9);
   M[2][1][1] = (0.125f * c3) * a2yz;
   M[1][2][1] = (0.125f * c3) * a2xz;
   M[1][1][2] = (0.125f * c3) * a2xy;
-  M[2][2][1] = 0.125f * fmaf(c9, a1z, c3 * (a3xxz + a3yyz));
-  M[2][1][2] = 0.125f * fmaf(c9, a1y, c3 * (a3xxy + a3yzz));
-  M[1][2][2] = 0.125f * fmaf(c9, a1x, c3 * (a3xyy + a3xzz));
-  M[2][2][2] = 0.125f * fmaf(c27, drho, c9 * (a2xx + a2yy + a2zz));
+  M[2][2][1] = fmaf(0.5f * c3, M[2][0][1] + M[0][2][1], -(0.25f * c9) * M[0][0][1]);
+  M[2][1][2] = fmaf(0.5f * c3, M[2][1][0] + M[0][1][2], -(0.25f * c9) * M[0][1][0]);
+  M[1][2][2] = fmaf(0.5f * c3, M[1][2][0] + M[1][0][2], -(0.25f * c9) * M[1][0][0]);
+  M[2][2][2] = fmaf(0.25f * c9, s200_020 + M[0][0][2], drho * (-1.f / 108.f));
   // pass z: G[a][b][cz]
   float G[3][3][3];
 #pragma unroll
