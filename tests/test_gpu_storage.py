@@ -24,6 +24,9 @@ pytestmark = pytest.mark.gpu
 
 # (rho, u, Pi) bounds per storage
 TOL = {"fp16": (5e-4, 5e-3, 5e-2), "bf16": (5e-3, 1e-1, 5e-1)}
+# the same at the C3 size, where Pi^neq is of the order of the velocity
+# rounding's gradients (test_c3_channel_storage_error)
+TOL_C3 = {"fp16": (5e-4, 5e-3, 0.3), "bf16": (5e-3, 0.1, 3.0)}
 
 
 def check_bounds(err, storage):
@@ -100,3 +103,35 @@ def test_droplet_reduced_storage_conserves_mass(storage):
     # storage rounding of rho-1 and phi each step: a random walk well below 1e-4
     assert abs(r1[0] - r0[0]) < 1e-4 * r0[0]
     assert abs(r1[1] - r0[1]) < 1e-2 * r0[1]
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+def test_c3_channel_storage_error(storage):
+    """The C3 size itself (channel 512x256x256, Re_tau 180, 200 steps):
+    reduced storage against the FP32 path, which tracks FP64 within 1e-5
+    (tests/test_gpu_parity.py).  Prints max and RMS errors per field.
+
+    Here Pi^neq (~nu du/dy ~ 4e-6) is of the order of the velocity gradients
+    that the rounding of the stored u makes (FP16 steps 6e-5 at u ~ 0.08,
+    BF16 5e-4), so Pi^neq carries that noise: the bounds on Pi are far above
+    the small cases' (measured FP16: rho 1.5e-4, u 2.0e-3, Pi 0.16 max; RMS
+    2.3e-5, 7.8e-4, 0.03-0.12)."""
+    case = configs.channel()
+    nz, ny, nx = case.shape
+    out = {}
+    for st in ("fp32", storage):
+        with Lattice(nx, ny, nz, storage=st, **case.lattice) as lat:
+            case.apply(lat)
+            lat.step(200)
+            out[st] = lat.download()
+    ref, got = out["fp32"], out[storage]
+    err = field_errors(got, ref)
+    rms = field_rms_errors(got, ref)
+    print(storage, "C3 max", {k: f"{v:.2e}" for k, v in err.items() if k != "phi"})
+    print(storage, "C3 rms", {k: f"{v:.2e}" for k, v in rms.items() if k != "phi"})
+    t_rho, t_u, t_pi = TOL_C3[storage]
+    assert err["rho"] <= t_rho, ("rho", err["rho"])
+    for k in ("u", "v", "w"):
+        assert err[k] <= t_u, (k, err[k])
+    for k in ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"):
+        assert err[k] <= t_pi, (k, err[k])
