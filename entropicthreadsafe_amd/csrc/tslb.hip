@@ -1565,21 +1565,23 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
 }
 
 // k_curv_list: curvature of the listed runs of tile-planes.  Per run a block
-// marches its planes as k_normals_tiled does: phi of planes z-2..z+2 on the
-// (TX+4)x(TY+4) footprint in a five-plane LDS ring (the plane three ahead
-// loads into registers meanwhile), n of planes z-1..z+1 on the (TX+2)x(TY+2)
-// footprint in a three-plane ring, kappa = -div(n) at the own nodes past the
-// gate -- the arithmetic of k_normals_tiled, so the values are the same bits
-// -- and kappa plus the flag byte written when any kappa of the plane is
-// nonzero.  Blocks stride over the list (its length read from the counter),
-// so every wave ends its loop.
+// marches its planes as k_normals_tiled does: phi of planes z-3..z+3 on the
+// (TX+4)x(TY+4) footprint in a six-plane LDS ring (the plane three ahead
+// loads into registers meanwhile), n of planes z-1..z+2 on the (TX+2)x(TY+2)
+// footprint in a four-plane ring, kappa = -div(n) at the own nodes past the
+// gate -- the arithmetic of k_normals_tiled, so the values are the same bits.
+// Every listed tile-plane gets its kappa (zeros included) and flag byte 1,
+// so one barrier per plane suffices.  Blocks stride over the list (its
+// length read from the counter), so every wave ends its loop.
 template <typename S>
 __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, float *__restrict__ kap,
                                                     uint8_t *__restrict__ flags, const int *__restrict__ cnt,
                                                     const int *__restrict__ list, const uint32_t *__restrict__ links,
                                                     int has_mask, Params p) {
-  __shared__ float ph[5][NPSLOT];
-  __shared__ float nl[3][3][NSLOT];
+  // rings deep enough that a slot is rewritten only two barriers after its
+  // last read, so one barrier per plane orders every LDS access
+  __shared__ float ph[6][NPSLOT];
+  __shared__ float nl[4][3][NSLOT];
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const size_t plane = (size_t)p.nx * p.ny;
@@ -1601,8 +1603,8 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
     };
     const uint32_t o0 = phi_off(tid), o1 = two ? phi_off(tid + NTHR) : 0u;
-    auto r5 = [zs](int z) { return (z - zs + 10) % 5; };
-    auto r3 = [zs](int z) { return (z - zs + 3) % 3; };
+    auto rph = [zs](int z) { return (z - zs + 12) % 6; };  // phi planes z >= zs - 2
+    auto rnl = [zs](int z) { return (z - zs + 4) & 3; };    // normal planes z >= zs - 1
     S qa{}, qb{};
     auto issue_to = [&](int z, S &ra, S &rb) {
       const rsrc_t r = rsrc(phi + plane * (size_t)clampw(z, p.nz, p.pz), pbS);
@@ -1616,8 +1618,8 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
         a *= p.isc[TSLB_PHI];
         c *= p.isc[TSLB_PHI];
       }
-      ph[r5(z)][tid] = a;
-      if (two) ph[r5(z)][tid + NTHR] = c;
+      ph[rph(z)][tid] = a;
+      if (two) ph[rph(z)][tid + NTHR] = c;
     };
     auto commit = [&](int z) { commit_from(z, qa, qb); };
     // n of plane zz from the in-plane sums of phi plane zz+1, the centre of
@@ -1630,8 +1632,8 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       int gz = 0;
       const bool zin = src_coord(zz, p.nz, p.pz, gz);
       const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
-      const float *P0 = ph[r5(zz - 1)], *P2 = ph[r5(zz + 1)];
-      float(*dn)[NSLOT] = nl[r3(zz)];
+      const float *P0 = ph[rph(zz - 1)], *P2 = ph[rph(zz + 1)];
+      float(*dn)[NSLOT] = nl[rnl(zz)];
       auto one = [&](int sl, int c, GradRun &g) {
         float nn[3];
         grad_run_step(g, plane_grad(P2, c), P0[c], nn);
@@ -1658,22 +1660,28 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
       for (int k = 0; k < 5; k++) commit_from(zs - 2 + k, pa[k], pb[k]);
     }
     __syncthreads();
-    grad_run_init(g0, plane_grad(ph[r5(zs - 2)], c0), plane_grad(ph[r5(zs - 1)], c0));
-    if (ntwo) grad_run_init(g1, plane_grad(ph[r5(zs - 2)], c1), plane_grad(ph[r5(zs - 1)], c1));
+    grad_run_init(g0, plane_grad(ph[rph(zs - 2)], c0), plane_grad(ph[rph(zs - 1)], c0));
+    if (ntwo) grad_run_init(g1, plane_grad(ph[rph(zs - 2)], c1), plane_grad(ph[rph(zs - 1)], c1));
     normals(zs - 1);
     normals(zs);
     for (int z = zs; z < zs + len; z++) {
       normals(z + 1);
+      // phi of plane z+3 into the slot of plane z-3, whose last reads (n of
+      // plane z-2, the gate of plane z-3) came before the previous barrier
+      if (z + 1 < zs + len) {
+        commit(z + 3);
+        if (z + 2 < zs + len) issue(z + 4);
+      }
       __syncthreads();
       float curv = 0.f;
       if (active) {
         const int own = (ty + 1) * SW + (tx + 1);
-        const float(*n0)[NSLOT] = nl[r3(z)];
+        const float(*n0)[NSLOT] = nl[rnl(z)];
         const float a = n0[0][own], bb = n0[1][own], c = n0[2][own];
-        const float f = ph[r5(z)][(ty + 2) * PW + (tx + 2)];
+        const float f = ph[rph(z)][(ty + 2) * PW + (tx + 2)];
         const float ind = f * (1.f - f) * (a * a + bb * bb + c * c);
         if (ind > 0.2f) {
-          const int zm = r3(z - 1), z0s = r3(z), zp = r3(z + 1);
+          const int zm = rnl(z - 1), z0s = rnl(z), zp = rnl(z + 1);
           auto nat = [&](int comp) {
             return [&, comp](int dx, int dy, int dz) -> float {
               const int zz = dz < 0 ? zm : (dz > 0 ? zp : z0s);
@@ -1686,14 +1694,10 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
           curv = -(dxx + dyy + dzz);
         }
       }
-      // phi of plane z+3 replaces z-2 (read last by n of plane z-1)
-      if (z + 1 < zs + len) {
-        commit(z + 3);
-        if (z + 2 < zs + len) issue(z + 4);
-      }
-      const int flag = __syncthreads_or(curv != 0.f);
-      if (flag && active) bst(rsrc(kap + plane * (size_t)z, pb4), (uint32_t)(x + p.nx * y) * 4u, curv);
-      if (tid == 0) flags[(size_t)z * tiles_plane + tile] = (uint8_t)flag;
+      // every listed tile-plane is flagged and gets its kappa (zeros
+      // included): no block-wide vote, no second barrier
+      if (active) bst(rsrc(kap + plane * (size_t)z, pb4), (uint32_t)(x + p.nx * y) * 4u, curv);
+      if (tid == 0) flags[(size_t)z * tiles_plane + tile] = 1;
     }
     __syncthreads();  // LDS reuse by the next entry
   }
