@@ -12,7 +12,10 @@ algorithmic bytes per launch (88 B per lattice update: 11 FP32 fields read
 once and written once) over its mean launch time, measured with HIP events on
 the library's stream inside the timed region.  `cpu_baseline` times the CPU
 oracle (the reference's population-push algorithm, FP32) on a slab of the same
-workload on this host's cores.
+workload on this host's cores.  Other workloads (--workload): the channel,
+the Taylor-Green vortex, the colour-gradient sphere (11 fields, its colour
+gradient formed inside the fused pass: 88 B per lattice update) and the
+closure LES.
 """
 import argparse
 import json
@@ -81,9 +84,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=512, help="lattice edge (512 for the named workload)")
-    ap.add_argument("--workload", default="emulsion", choices=("emulsion", "channel", "tgv", "closure-les"),
+    ap.add_argument("--n", type=int, default=None,
+                    help="lattice edge (default 512; 256 for the colour sphere)")
+    ap.add_argument("--workload", default="emulsion",
+                    choices=("emulsion", "channel", "tgv", "colour", "closure-les"),
                     help="emulsion (2C, the named workload), channel (1C, n x n/2 x n/2), tgv (1C, n^3), "
+                         "colour (colour-gradient sphere, 256^3), "
                          "closure-les (learned closure on n/4 against the filtered n^3 run)")
     ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16", "bf16"),
                     help="moment storage precision (compute is FP32)")
@@ -101,12 +107,15 @@ def main():
     if args.workload == "closure-les":
         closure_les(args)
         return
-    n = args.n
+    n = args.n or (256 if args.workload == "colour" else 512)
     t0 = time.time()
     if args.workload == "emulsion":
         case = configs.emulsion(n=n)
     elif args.workload == "channel":
         case = configs.channel(nx=n, ny=n // 2, nz=n // 2)
+    elif args.workload == "colour":
+        # colour-gradient sphere of radius 40 at 256^3 (sigma 0.02, beta 0.7)
+        case = configs.colour_droplet(n=n, radius=40.0 * n / 256, sigma=0.02, beta=0.7)
     else:
         case = configs.tgv(n=n)
     nz, ny, nx = case.shape
@@ -136,7 +145,7 @@ def main():
         f"phi drift {(r1[1] - r0[1]) / max(r0[1], 1e-30):.2e}")
 
     mlups = nodes * args.steps / wall / 1e6
-    nfields = 11 if ncomp == 2 else 10
+    nfields = 11 if ncomp >= 2 else 10
     es = 4 if args.storage == "fp32" else 2
     bytes_per_launch = 2.0 * es * nfields * nodes     # read + write of the stored moment fields
     t_fused = fused_ms / max(fused_n, 1) / 1e3
@@ -186,7 +195,7 @@ def closure_les(args):
     energy-spectrum errors of the coarse runs (closure and plain) against the
     filtered fine run."""
     from entropicthreadsafe_amd import les
-    res = les.run(n=args.n, log=log)
+    res = les.run(n=args.n or 512, log=log)
     out = {
         "metric": "closure-LES speedup at fixed error",
         "value": round(res["speedup_closure_vs_fine"], 2),

@@ -417,15 +417,23 @@ __device__ __forceinline__ void grad_run_init(GradRun &g, const PlaneGrad &a, co
   g.sm = a.s;
   g.sz = b.s;
 }
-// the unit normal of plane z from the sums of plane z+1 and the centre value
-// of plane z-1 (pcm), and the advance to z+1; the gradient is kept as
-// 72 grad(phi), so the 1e-9 floor becomes 7.2e-8
+// the unit normal (NORM) or the gradient of plane z from the sums of plane
+// z+1 and the centre value of plane z-1 (pcm), and the advance to z+1; the
+// sums give 72 grad(phi), so the 1e-9 floor of the normal becomes 7.2e-8
+template <bool NORM = true>
 __device__ __forceinline__ void grad_run_step(GradRun &g, const PlaneGrad &n, float pcm, float *nn) {
   const float gx = g.rx + n.x, gy = g.ry + n.y, gz = fmaf(36.f, n.c - pcm, n.s - g.sm);
-  const float inv = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(gx, gx, fmaf(gy, gy, gz * gz))) + 7.2e-8f);
-  nn[0] = gx * inv;
-  nn[1] = gy * inv;
-  nn[2] = gz * inv;
+  if constexpr (NORM) {
+    const float inv = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaf(gx, gx, fmaf(gy, gy, gz * gz))) + 7.2e-8f);
+    nn[0] = gx * inv;
+    nn[1] = gy * inv;
+    nn[2] = gz * inv;
+  } else {
+    constexpr float k = 1.f / 72.f;
+    nn[0] = gx * k;
+    nn[1] = gy * k;
+    nn[2] = gz * k;
+  }
   g.rx = fmaf(4.f, n.x, g.lx);
   g.lx = n.x;
   g.ry = fmaf(4.f, n.y, g.ly);
@@ -433,15 +441,16 @@ __device__ __forceinline__ void grad_run_step(GradRun &g, const PlaneGrad &n, fl
   g.sm = g.sz;
   g.sz = n.s;
 }
-// the unit normal at centre c from phi planes z-1, z, z+1: the operations of
-// the march in the same order, so every pass gets the same bits
-template <int PITCH = PW>
+// the unit normal (NORM) or the gradient at centre c from phi planes z-1, z,
+// z+1: the operations of the march in the same order, so every pass gets
+// the same bits
+template <int PITCH = PW, bool NORM = true>
 __device__ __forceinline__ void sep_normal(const float *P0, const float *P1, const float *P2, int c,
                                            float &n0, float &n1, float &n2) {
   GradRun g;
   grad_run_init(g, plane_grad<PITCH>(P0, c), plane_grad<PITCH>(P1, c));
   float nn[3];
-  grad_run_step(g, plane_grad<PITCH>(P2, c), P0[c], nn);
+  grad_run_step<NORM>(g, plane_grad<PITCH>(P2, c), P0[c], nn);
   n0 = nn[0];
   n1 = nn[1];
   n2 = nn[2];
@@ -559,14 +568,9 @@ __device__ __forceinline__ void load_m32(const PlaneSrc &ps, uint32_t offS, Src<
   for (int q = 0; q < (NC == 3 ? 11 : 10); q++) s.m[q] = bld<float>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
 }
 
-// The other loads of one source: colour gradient, gated curvature, closure.
+// The other loads of one source: gated curvature, closure.
 template <int NC, bool CLO>
 __device__ __forceinline__ void load_extra(const PlaneSrc &ps, uint32_t off4, uint32_t offC, Src<NC, CLO> &s) {
-  if constexpr (NC == 3) {  // colour gradient
-    s.nn[0] = bld<float>(rsrc(ps.n, ps.pb4), off4);
-    s.nn[1] = bld<float>(rsrc(ps.n + ps.fs4, ps.pb4), off4);
-    s.nn[2] = bld<float>(rsrc(ps.n + 2 * ps.fs4, ps.pb4), off4);
-  }
   // phase field: the gated curvature (the normals are computed by the caller)
   if constexpr (NC == 2) s.kap = bld<float>(rsrc(ps.n + 3 * ps.fs4, ps.pb4), off4);
   if constexpr (CLO) {
@@ -844,8 +848,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   constexpr int NF = NC >= 2 ? 11 : 10;
   __shared__ float lds[NQ][NSLOT];
   // phase field: phi planes zs-1, zs, zs+1 of the (TX+4)x(TY+4) footprint
-  constexpr int NPR = NC == 2 ? 3 : 1;
-  __shared__ float phr[NPR][NC == 2 ? NPSLOT : 1];
+  constexpr int NPR = NC >= 2 ? 3 : 1;
+  __shared__ float phr[NPR][NC >= 2 ? NPSLOT : 1];
   __shared__ uint64_t ilist_wm[NTHR / 64];
   __shared__ int ilist_base;
 
@@ -888,13 +892,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 
   // phase field: phi footprint slots tid and tid + NTHR of this thread, and
   // their in-plane byte offsets (nearest interior node across a wall)
-  const bool ptwo = NC == 2 && tid + NTHR < NPSLOT;
+  const bool ptwo = NC >= 2 && tid + NTHR < NPSLOT;
   auto phi_off = [&](int sl) -> uint32_t {
     const int gx = clampw(x0 + sl % PW - 2, p.nx, p.px);
     const int gy = clampw(y0 + sl / PW - 2, p.ny, p.py);
     return (uint32_t)(gx + p.nx * gy) * (uint32_t)sizeof(S);
   };
-  const uint32_t pofs0 = NC == 2 ? phi_off(tid) : 0u, pofs1 = ptwo ? phi_off(tid + NTHR) : 0u;
+  const uint32_t pofs0 = NC >= 2 ? phi_off(tid) : 0u, pofs1 = ptwo ? phi_off(tid + NTHR) : 0u;
   S pr0{}, pr1{};  // raw storage values, converted at commit
   auto phi_issue = [&](int z) {
     const rsrc_t r = rsrc((const char *)src + TSLB_PHI * fsS + (size_t)clampw(z, p.nz, p.pz) * pbS, pbS);
@@ -996,7 +1000,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   // and zs+1; plane zs+2 is loaded meanwhile and replaces zs-1 in pm.
   auto body = [&](int zs, Acc &aA, Acc &aB, Acc &aC, uint32_t &bmA, uint32_t &bmB, uint32_t &bmC,
                   float *pm, const float *p0, const float *pp) {
-    if constexpr (NC == 2) {
+    if constexpr (NC >= 2) {
       if (zs + 2 <= z1 + 1) phi_issue(zs + 2);  // planes up to z1 + 1 are read
     }
     Src<NC, CLO> co, cr;
@@ -1006,14 +1010,15 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
     float gax = 0.f, gay = 0.f, gb = 0.f;  // phase field: own A_x, A_y, phi/8
     float oc[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, rc[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // colour: cA, bc, g
+    // phase field: unit normals; colour model: the colour gradient
     float nno[3], nnr[3];
-    if constexpr (NC == 2) {
-      grad_run_step(gro, plane_grad(pp, co_c), pm[co_c], nno);
-      if (tid < NRING) grad_run_step(grr, plane_grad(pp, cr_c), pm[cr_c], nnr);
+    if constexpr (NC >= 2) {
+      grad_run_step<NC == 2>(gro, plane_grad(pp, co_c), pm[co_c], nno);
+      if (tid < NRING) grad_run_step<NC == 2>(grr, plane_grad(pp, cr_c), pm[cr_c], nnr);
     }
     // ---- own source slot
     if (co_ok) {
-      if constexpr (NC == 2) {
+      if constexpr (NC >= 2) {
         co.nn[0] = nno[0];
         co.nn[1] = nno[1];
         co.nn[2] = nno[2];
@@ -1037,7 +1042,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     }
     // ---- halo ring slot
     if (cr_ok) {
-      if constexpr (NC == 2) {
+      if constexpr (NC >= 2) {
         cr.nn[0] = nnr[0];
         cr.nn[1] = nnr[1];
         cr.nn[2] = nnr[2];
@@ -1049,7 +1054,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         for (int k = 0; k < 5; k++) rc[k] = gq[k];
     }
     __syncthreads();
-    if constexpr (NC == 2) phi_commit(pm);  // read again after the closing barrier
+    if constexpr (NC >= 2) phi_commit(pm);  // read again after the closing barrier
 
     bmC = plane_mask(zs + 1);
     acc_zero(aC);
@@ -1178,7 +1183,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
   acc_zero(a1);
   uint32_t m0 = 0, m1 = plane_mask(z0 - 1), m2 = 0;
   float *R0 = phr[0], *R1 = phr[NPR > 1 ? 1 : 0], *R2 = phr[NPR > 2 ? 2 : 0];
-  if constexpr (NC == 2) {
+  if constexpr (NC >= 2) {
     for (int k = 0; k < 3; k++) {
       phi_issue(z0 - 2 + k);
       phi_commit(phr[k < NPR ? k : 0]);
@@ -1361,19 +1366,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
       if (!zero && zin && okxy) {
         bool ok = true;
         if (has_mask) ok = (bld_u32(lk, xy) & kSolid) == 0u;
-        if (ok) {
-          auto get = [&](int dx, int dy, int dz) -> float {
-            const float *pl = dz < 0 ? P0 : (dz > 0 ? P2 : P1);
-            return pl[c + dy * PW + dx];
-          };
-          if constexpr (RAW) {
-            a = grad_axis<0>(get);
-            bb = grad_axis<1>(get);
-            cc = grad_axis<2>(get);
-          } else {
-            sep_normal(P0, P1, P2, c, a, bb, cc);
-          }
-        }
+        if (ok) sep_normal<PW, !RAW>(P0, P1, P2, c, a, bb, cc);
       }
       dn[0][sl] = a;
       dn[1][sl] = bb;
@@ -2791,6 +2784,10 @@ int tslb_normals(tslb_ctx *c) {
   int rc = check_ctx_step(c);
   if (rc) return rc;
   if (c->prm.ncomp < 2) return TSLB_OK;
+  if (c->prm.ncomp == TSLB_COLOUR) {  // the fused pass forms the colour gradient itself
+    c->normals_fresh = 1;
+    return TSLB_OK;
+  }
   return launch_normals(c);
 }
 
@@ -2808,7 +2805,7 @@ int tslb_step(tslb_ctx *c, int nsteps) {
   if (nsteps < 0) return fail(TSLB_E_ARG, "nsteps < 0");
   for (int s = 0; s < nsteps; s++) {
     if (c->clo_mode == 2 && (rc = launch_closure(c))) return rc;
-    if (c->prm.ncomp >= 2 && (rc = launch_normals(c))) return rc;
+    if (c->prm.ncomp == 2 && (rc = launch_normals(c))) return rc;
     if ((rc = launch_fused(c))) return rc;
     if ((rc = launch_zbc(c))) return rc;
   }
