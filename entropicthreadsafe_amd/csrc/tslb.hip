@@ -22,9 +22,10 @@
 //               24 in-plane-moving ones (+2 phase fluxes) through LDS, and each
 //               destination column gathers its populations and accumulates
 //               moment sums for three destination planes in registers.  Phase
-//               field: the normals come from three phi planes kept in an LDS
-//               ring.
-//   k_normals   colour gradient (colour model) or sparse curvature (phase field).
+//               field and colour model: the normals or the colour gradient
+//               come from three phi (psi) planes kept in an LDS ring.
+//   k_normals   dense normals and curvature or colour gradient (downloads,
+//               boundary passes); sparse curvature (phase-field step).
 //   k_prepush   initial one-step advection of phi (reference writes g pushed).
 //   k_links     per-node bounce-back link masks from an isfluid map.
 //   k_zplane_bc moment-space inlet/outlet planes.
