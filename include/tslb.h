@@ -127,8 +127,10 @@ int tslb_checkpoint_write(tslb_ctx *ctx, const char *path, int64_t step);
 int tslb_checkpoint_read(tslb_ctx *ctx, const char *path, int64_t *step);
 
 /* Colour-gradient normals n = grad(phi)/(|grad(phi)|+1e-9) of the current
- * state, 3*nx*ny*nz floats (nx, ny, nz components).  Valid after a step or
- * after tslb_normals().  Replaces `!$acc update host(normx,...)`. */
+ * state, 3*nx*ny*nz floats (nx, ny, nz components); for ncomp =
+ * TSLB_COLOUR the unnormalised colour gradient grad(psi).  Computed over the
+ * whole lattice from the current state when called (the same bits the step
+ * uses).  Replaces `!$acc update host(normx,...)`. */
 int tslb_download_normals(tslb_ctx *ctx, float *n3);
 
 /* Initial state from the uploaded rho, u, v, w, phi: Pi^neq = 0 (f = feq at
@@ -146,7 +148,10 @@ int tslb_init_equilibrium(tslb_ctx *ctx, int prepush_phi);
 int tslb_step(tslb_ctx *ctx, int nsteps);
 
 /* Split entry points, for bisection against the reference's kernel order. */
-/* K2 colour gradient (recursive_2c_TSLB3D.f90:543-577). */
+/* K2 colour gradient (recursive_2c_TSLB3D.f90:543-577): for the phase field
+ * the curvature pass of the step (the fused pass forms the normals itself);
+ * for ncomp = TSLB_COLOUR nothing runs (the fused pass forms the colour
+ * gradient), so the split sequence is the same for every model. */
 int tslb_normals(tslb_ctx *ctx);
 /* K3 fused collide + stream followed by K1 moments of the streamed
  * populations, as one moment-space pass (recursive_2c_TSLB3D.f90:607-836 then
