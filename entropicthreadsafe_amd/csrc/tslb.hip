@@ -544,29 +544,31 @@ struct PlaneSrc {
 // kOOB where the tile-plane has no interface (the buffer then returns 0
 // without a memory access).
 constexpr uint32_t kOOB = 0x80000000u;
+// The ten hydrodynamic moments of a source are loaded; phi (psi) of the two
+// component models comes from the fused pass's LDS ring instead.
 // Reduced storage: the raw values of one source, loaded before any is
 // converted (left to itself the scheduler reuses one register per load and
 // waits out each round trip).
 template <int NC, typename S>
 struct RawSrc {
-  S v[NC == 3 ? 11 : 10];
+  S v[10];
 };
 template <int NC, typename S>
 __device__ __forceinline__ void load_raw(const PlaneSrc &ps, uint32_t offS, RawSrc<NC, S> &r) {
 #pragma unroll
-  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) r.v[q] = bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
+  for (int q = 0; q < 10; q++) r.v[q] = bld<S>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
 }
 template <int NC, typename S, bool CLO>
 __device__ __forceinline__ void convert_raw(const RawSrc<NC, S> &r, const Params &p, Src<NC, CLO> &s) {
 #pragma unroll
-  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) s.m[q] = to_f(r.v[q]) * p.isc[q];
+  for (int q = 0; q < 10; q++) s.m[q] = to_f(r.v[q]) * p.isc[q];
 }
 
 // FP32 storage: the moments of one source straight into place.
 template <int NC, bool CLO>
 __device__ __forceinline__ void load_m32(const PlaneSrc &ps, uint32_t offS, Src<NC, CLO> &s) {
 #pragma unroll
-  for (int q = 0; q < (NC == 3 ? 11 : 10); q++) s.m[q] = bld<float>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
+  for (int q = 0; q < 10; q++) s.m[q] = bld<float>(rsrc(ps.f + q * ps.fsS, ps.pbS), offS);
 }
 
 // The other loads of one source: gated curvature, closure.
