@@ -1235,14 +1235,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
 // the current plane is computed.  RAW (colour model) writes the unnormalised
 // gradient of the colour field psi and no curvature.
 //
-// Sparse output (phase field): the fused pass computes n itself from phi and
-// needs only kappa, which is zero outside a thin shell round the interface;
-// each (tile, plane) gets a flag byte, 1 when any of its kappa is nonzero, and
-// kappa is written only for flagged tile-planes, where the fused pass reads
-// it.  A plane whose phi window z-1..z+1 is uniform skips the stencil;
-// uniformity of each loaded plane is voted per wave from registers.  `dense`
-// writes n and kappa everywhere (and flags every tile-plane); RAW is always
-// dense.
+// The library launches it dense: `dense` writes n and kappa everywhere (and
+// flags every tile-plane), RAW always is.  A plane whose phi window z-1..z+1
+// is uniform skips the stencil; uniformity of each loaded plane is voted per
+// wave from registers.  Its sparse mode (kappa only, each (tile, plane) with
+// a flag byte, 1 when any of its kappa is nonzero, gated windows skipped)
+// ran the phase-field step before the run-list pair k_iface_list +
+// k_curv_list replaced it.
 constexpr int NWAVE = NTHR / 64;
 template <typename S, bool RAW>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_normals_tiled(const S *__restrict__ phi,
