@@ -212,6 +212,8 @@ def closure_les(args):
         "config": {"workload": res["workload"], "factor": res["factor"], "kmax": res["kmax"],
                    "train_steps": res["train_steps"], "batch": res["batch"], "samples": res["samples"]},
         "closure_spectrum_error": res["closure_spectrum_error"],
+        "balanced_spectrum_error": res.get("balanced_spectrum_error"),
+        "nu_t_balance": res.get("nu_t_balance"),
         "plain_spectrum_error": res["plain_spectrum_error"],
         "fine_seconds": round(res["fine_seconds"], 4),
         "closure_seconds": round(res["closure_seconds"], 4),

@@ -115,10 +115,12 @@ def _timed_steps(lat, steps):
 
 
 def run(n=512, r=4, k0=4.0, urms=0.04, spinup=1500, snaps=4, snap_every=250, horizon=2000,
-        train_steps=9000, batch=16384, lr=2e-3, seed=1, drops=True, log=print):
+        train_steps=9000, batch=16384, lr=2e-3, seed=1, drops=True, balance=False, log=print):
     """The experiment; returns a dict of its results.  drops=False runs the
     same two-component lattice without droplets (phi = 0): single-phase
-    turbulence, a check of the coarse-grid scaling."""
+    turbulence, a check of the coarse-grid scaling.  The mean subgrid
+    dissipation of the data and of the trained model is always recorded;
+    balance=True adds a coarse run with nu_t scaled to equalise them."""
     case = configs.emulsion(n=n)
     kw = dict(case.lattice)
     nc = n // r
@@ -166,20 +168,40 @@ def run(n=512, r=4, k0=4.0, urms=0.04, spinup=1500, snaps=4, snap_every=250, hor
                    batch=batch, t_scale=t_scale, samples=model.num_samples)
         log(f"les: training loss {l0:.4g} -> {l1:.4g} ({train_steps} Adam steps in {t_train:.1f} s)")
 
+        # energy balance over the samples: the factor on nu_t that makes the
+        # model's mean subgrid dissipation <2 rho nu_t S:S - dsigma:S> equal
+        # the filtered data's <-tau:S> (off-diagonal pairs counted twice)
+        out = model.forward().astype(np.float64)
+        S = feat[:6].astype(np.float64)
+        c = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])[:, None]
+        eps_true = float(-(c * aux[1:].astype(np.float64) * S).sum(0).mean())
+        eps_nu = float((2.0 * aux[0].astype(np.float64) * out[0] * (c * S * S).sum(0)).mean())
+        eps_ds = float(-(c * out[1:] * S).sum(0).mean())
+        alpha = (eps_true - eps_ds) / eps_nu if eps_nu > 0 else 0.0
+        del out, S
+        res.update(eps_subgrid=eps_true, eps_model=eps_nu + eps_ds, nu_t_balance=alpha)
+        log(f"les: subgrid dissipation {eps_true:.4g}, model {eps_nu + eps_ds:.4g}; nu_t factor {alpha:.4g}")
+
         # the coarse runs
         spectra = {}
-        for name in ("plain", "closure"):
+        theta = model.get_params()
+        names = ("plain", "closure") + (("balanced",) if balance and alpha > 0 else ())
+        for name in names:
+            if name == "balanced":
+                tb = theta.copy()
+                tb[-7] *= alpha            # the nu_t output scale
+                model.set_params(tb)
             with _coarse_lattice(nc, kw, r, rb, vel, phi) as lat:
-                if name == "closure":
+                if name != "plain":
                     lat.set_closure(model)
                     clo = lat.download_closure()   # fields of the t0 state
-                    res["closure_nu_t_mean"] = float(clo[0].mean())
-                    res["closure_nu_t_max"] = float(clo[0].max())
-                    res["closure_dsigma_rms"] = float(np.sqrt((clo[1:].astype(np.float64) ** 2).mean()))
+                    res[f"{name}_nu_t_mean"] = float(clo[0].mean())
+                    res[f"{name}_nu_t_max"] = float(clo[0].max())
+                    res[f"{name}_dsigma_rms"] = float(np.sqrt((clo[1:].astype(np.float64) ** 2).mean()))
                     res["nu_coarse"] = (0.5 + (kw["tau"] - 0.5) / r - 0.5) / 3.0
                 dt = _timed_steps(lat, horizon // r)
                 m = lat.download()
-                if name == "closure":
+                if name != "plain":
                     lat.set_closure(None)
             ok = bool(np.isfinite(m).all())
             spectra[name] = energy_spectrum(m[1], m[2], m[3]) if ok else None
