@@ -1008,12 +1008,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     }
     Src<NC, CLO> co, cr;
     bool co_ok, cr_ok;
-    // two components: the plane's loads issue at raised wave priority, ahead
-    // of the other block's arithmetic, so that their latency starts counting
-    // early (2C emulsion 2.66 -> 2.63 ms; one component 0.8 % slower, left out)
-    if constexpr (NC == 2) __builtin_amdgcn_s_setprio(2);
+    // two-component models: the plane's loads issue at raised wave priority,
+    // ahead of the other block's arithmetic, so that their latency starts
+    // counting early (2C emulsion 2.66 -> 2.63 ms, colour sphere -0.8 %; one
+    // component 0.8 % slower, left out)
+    if constexpr (NC >= 2) __builtin_amdgcn_s_setprio(2);
     fetch(zs, co, co_ok, cr, cr_ok);
-    if constexpr (NC == 2) __builtin_amdgcn_s_setprio(0);
+    if constexpr (NC >= 2) __builtin_amdgcn_s_setprio(0);
     float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
     float gax = 0.f, gay = 0.f, gb = 0.f;  // phase field: own A_x, A_y, phi/8
