@@ -1015,12 +1015,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     if constexpr (NC >= 2) __builtin_amdgcn_s_setprio(2);
     fetch(zs, co, co_ok, cr, cr_ok);
     // the waves that also evaluate ring sources keep priority 1 until the
-    // barrier, the others drop to 0
+    // barrier (they set when it releases), the others drop to 0
     const bool ring_wave = (tid >> 6) < (NRING + 63) / 64;
-    if constexpr (NC >= 2) {
-      if (ring_wave) __builtin_amdgcn_s_setprio(1);
-      else __builtin_amdgcn_s_setprio(0);
-    }
+    if (ring_wave) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
     float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
     float gax = 0.f, gay = 0.f, gb = 0.f;  // phase field: own A_x, A_y, phi/8
@@ -1068,7 +1066,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       if constexpr (NC == 3)
         for (int k = 0; k < 5; k++) rc[k] = gq[k];
     }
-    if constexpr (NC >= 2) __builtin_amdgcn_s_setprio(0);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     if constexpr (NC >= 2) phi_commit(pm);  // read again after the closing barrier
 
