@@ -1116,6 +1116,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       // second LDS pass: every source slot turns its total populations into
       // the recoloured A populations, and destinations sum those
       __syncthreads();
+      if (ring_wave) __builtin_amdgcn_s_setprio(1);  // the ring rewrites set this barrier too
       float a0 = 0.f, a5 = 0.f, a6 = 0.f;
       if (co_ok) {
         colour_rewrite<1>(lds, (ty + 1) * SW + (tx + 1), oc);
@@ -1124,6 +1125,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         a6 = colour_a<6>(h6, oc[0], oc[1], oc[2], oc[3], oc[4]);
       }
       if (cr_ok) colour_rewrite<1>(lds, rsy * SW + rsx, rc);
+      __builtin_amdgcn_s_setprio(0);
       __syncthreads();
       if (active) {
         if ((bmA | bmB | bmC) == 0u) {
