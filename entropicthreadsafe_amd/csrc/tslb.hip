@@ -1011,16 +1011,18 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     // two-component models in FP32 storage: the plane's loads issue at raised
     // wave priority, ahead of the other block's arithmetic, so that their
     // latency starts counting early (2C emulsion 2.66 -> 2.63 ms, colour
-    // sphere -0.8 %; one component 0.8 % slower, left out).  Neither this
-    // hint nor the ring-wave one below pays with 16-bit storage, whose loads
-    // are issued in one batch before any conversion (2C emulsion without
-    // them: FP16 2.67 -> 2.60 ms, BF16 2.75 -> 2.62 ms)
-    if constexpr (NC >= 2 && sizeof(S) == 4) __builtin_amdgcn_s_setprio(2);
+    // sphere -0.8 %; one component 0.8 % slower, left out).  With 16-bit
+    // storage, whose loads are issued in one batch before any conversion,
+    // neither this hint nor the ring-wave one below pays for the phase-field
+    // model or one component (2C emulsion without them: FP16 2.67 -> 2.60 ms,
+    // BF16 2.75 -> 2.62 ms; channel BF16 0.58 -> 0.54 ms); the colour model
+    // keeps both (FP16 sphere 0.459 -> 0.448 ms with them)
+    if constexpr (NC >= 2 && (sizeof(S) == 4 || NC == 3)) __builtin_amdgcn_s_setprio(2);
     fetch(zs, co, co_ok, cr, cr_ok);
     // the waves that also evaluate ring sources keep priority 1 until the
-    // barrier (they set when it releases), the others drop to 0 (FP32 storage)
+    // barrier (they set when it releases), the others drop to 0
     const bool ring_wave = (tid >> 6) < (NRING + 63) / 64;
-    if (ring_wave && sizeof(S) == 4) __builtin_amdgcn_s_setprio(1);
+    if (ring_wave && (sizeof(S) == 4 || NC == 3)) __builtin_amdgcn_s_setprio(1);
     else __builtin_amdgcn_s_setprio(0);
     float gq[7];
     float h0 = 0.f, h5 = 0.f, h6 = 0.f, g0 = 0.f, g5 = 0.f, g6 = 0.f;
