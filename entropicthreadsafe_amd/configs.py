@@ -178,6 +178,7 @@ def place_droplets(n, ndrop, rmin, rmax, seed_l=7):
     """
     from . import tslb as _t  # the generator of the library (host C)
     drops = []
+    acc = np.zeros((0, 4))
     m = 0
     while len(drops) < ndrop and m < 178 * 178:
         a, b = m % 178 + 1, m // 178 + 1
@@ -186,15 +187,14 @@ def place_droplets(n, ndrop, rmin, rmax, seed_l=7):
         cy = _t.rand_noseeded(a, b, 2, seed_l) * n
         cz = _t.rand_noseeded(a, b, 3, seed_l) * n
         r = rmin + (rmax - rmin) * _t.rand_noseeded(a, b, 4, seed_l)
-        ok = True
-        for (x, y, z, rr) in drops:
-            d = np.array([cx - x, cy - y, cz - z])
+        if len(drops):
+            # minimum-image distance to every accepted droplet
+            d = np.array([cx, cy, cz]) - acc[:, :3]
             d = d - n * np.round(d / n)
-            if np.sqrt((d ** 2).sum()) < r + rr + 2.0:
-                ok = False
-                break
-        if ok:
-            drops.append((cx, cy, cz, r))
+            if (np.sqrt((d ** 2).sum(axis=1)) < r + acc[:, 3] + 2.0).any():
+                continue
+        drops.append((cx, cy, cz, r))
+        acc = np.vstack([acc, [cx, cy, cz, r]])
     return drops
 
 
@@ -230,3 +230,58 @@ def emulsion(n=512, ndrop=200, rmin=8.0, rmax=16.0, sigma=0.01, beta=0.7, tau=0.
                 dict(rho=np.ones_like(z), u=z, v=z, w=z, phi=phi),
                 noise=[("u", noise, 11), ("v", noise, 22), ("w", noise, 33)],
                 notes=f"{len(drops)} droplets, volume fraction {vol:.4f}")
+
+
+# ---------------------------------------------------------------------------
+# newBCS turbulent jet (TSLB_newBCS/recursive_TSbcs_TSLB3.f90,
+# TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292)
+# ---------------------------------------------------------------------------
+def _inflow_disk(nx, ny, radius):
+    j, i = np.meshgrid(np.arange(1, ny + 1), np.arange(1, nx + 1), indexing="ij")
+    return (i - nx / 2.0) ** 2 + (j - ny / 2.0) ** 2 <= radius ** 2
+
+
+def jet_inflow_initial(nx, ny, uwall=0.05, radius=10.0):
+    """Initial inflow w of plane k = 1, shape (ny, nx): inside the disk
+    uwall + 0.02 sqrt(-2 ln r_y) cos(2 pi r_x) with r_x, r_y the generator at
+    (i, j, 1, 524) and (i, j, 1, 1732) (recursive_TSbcs_TSLB3.f90:108-125), 0
+    outside."""
+    rx = rand_noseeded_grid(nx, ny, 1, 524)[0]
+    ry = rand_noseeded_grid(nx, ny, 1, 1732)[0]
+    two_pi = np.float32(2.0 * np.pi)
+    ry = np.maximum(ry, np.float32(1e-30))
+    w = np.float32(uwall) + np.float32(0.02) * np.sqrt(np.float32(-2.0) * np.log(ry)) * np.cos(two_pi * rx)
+    return np.where(_inflow_disk(nx, ny, radius), w, np.float32(0.0)).astype(np.float32)
+
+
+def jet_inflow_redraw(w_prev, step, uwall=0.05, radius=10.0):
+    """The inflow plane after the redraw of `step` (done when step % 10 == 0,
+    boundary_cds_3D_module.f90:1237-1259): inside the disk
+    uwall + 0.004 sqrt(-2 ln r) cos(2 pi r) with the same r = generator at
+    (i, j, 1, step) twice, as the reference; outside, w_prev is kept."""
+    ny, nx = w_prev.shape
+    r = rand_noseeded_grid(nx, ny, 1, step)[0]
+    rl = np.maximum(r, np.float32(1e-30))
+    two_pi = np.float32(2.0 * np.pi)
+    w = np.float32(uwall) + np.float32(0.004) * np.sqrt(np.float32(-2.0) * np.log(rl)) * np.cos(two_pi * r)
+    return np.where(_inflow_disk(nx, ny, radius), w, w_prev).astype(np.float32)
+
+
+def jet(nx=64, ny=64, nz=64, tau=0.6, uwall=0.05, ncomp=1, sigma=0.0, beta=0.7):
+    """newBCS jet: x, y periodic; planes k = 1 and k = nz carry the moment-space
+    inlet/outlet BC; u = v = w = 0, rho = 1 except w of plane k = 1, the
+    initial inflow disk (radius 10).  Two components add phi = 1 on the disk of
+    plane k = 1 (the inflow phase; no reference counterpart).  The caller
+    applies the BC with `jet_inflow_initial` and redraws with
+    `jet_inflow_redraw` (Lattice.set_zplane_bc)."""
+    z = np.zeros((nz, ny, nx), dtype=np.float64)
+    w = z.copy()
+    w_in = jet_inflow_initial(nx, ny, uwall)
+    w[0] = w_in
+    phi = z.copy()
+    kw = dict(ncomp=ncomp, tau=tau, periodic=(1, 1, 0))
+    if ncomp == 2:
+        phi[0] = _inflow_disk(nx, ny, 10.0)
+        kw.update(sigma=sigma, sharp_c=3.0 * beta)
+    return Case("jet", (nz, ny, nx), kw, dict(rho=np.ones_like(z), u=z, v=z.copy(), w=w, phi=phi),
+                prepush_phi=ncomp == 2, notes=f"jet inflow uwall={uwall}")

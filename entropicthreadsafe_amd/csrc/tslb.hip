@@ -2812,7 +2812,9 @@ int tslb_collide_stream(tslb_ctx *c) {
   if (rc) return rc;
   if (c->prm.ncomp >= 2 && !c->normals_fresh) return fail(TSLB_E_STATE, "tslb_normals() must precede tslb_collide_stream()");
   if (c->clo_mode == 2 && (rc = launch_closure(c))) return rc;
-  return launch_fused(c);
+  // the same fused pass and boundary planes as one tslb_step
+  if ((rc = launch_fused(c))) return rc;
+  return launch_zbc(c);
 }
 
 int tslb_step(tslb_ctx *c, int nsteps) {

@@ -124,8 +124,10 @@ program tslb_driver
       u(:, j, :) = real(1.5d0 * u0 * ((j - 0.5d0) / (ny / 2.d0)) * (2.d0 - (j - 0.5d0) / (ny / 2.d0)))
     enddo
   case ('jet')
-    ! inflow disk of radius 10 at z = 1 (boundary_cds_3D_module.f90:1233-1292)
-    call inflow_plane(0)
+    ! newBCS jet: inflow disk of radius 10 on k = 1, drawn with amplitude 0.02
+    ! from the generator at (i, j, 1, 524) and (i, j, 1, 1732)
+    ! (TSLB_newBCS/recursive_TSbcs_TSLB3.f90:108-125)
+    call inflow_initial()
   case ('jet_root')
     ! walls on all faces and an inflow disk of radius 20 on k = 1
     ! (recursive_2c_TSLB3D.f90:79-104); the seeded generator replaces
@@ -185,6 +187,17 @@ program tslb_driver
   if (dumpstep > 0) call execute_command_line('mkdir -p dump')
   ts1 = wall_seconds()
   do step = step0 + 1, step0 + nsteps
+    if (jet .and. mod(step, 10) == 0) then
+      ! newBCS order: moments, then the BC with the inflow redraw of this step,
+      ! then output and the collision (TSLB_newBCS/recursive_TSbcs_TSLB3.f90:
+      ! 563-617; boundary_cds_3D_module.f90:1237-1259).  The state here is the
+      ! moments of this step with the last BC applied; set_zplane_bc applies
+      ! the BC again with the new plane.
+      call start_timing2('LB', 'bcs_inflow')
+      call inflow_plane(step)
+      call tslb_check(tslb_set_zplane_bc(ctx, c_loc(w_in), c_loc(phi_in)), 'tslb_set_zplane_bc')
+      call end_timing2('LB', 'bcs_inflow')
+    endif
     if (mod(step, stamp2D) == 0) then
       call start_timing2('IO', 'print2d')
       iframe2D = iframe2D + 1
@@ -212,12 +225,6 @@ program tslb_driver
     call start_timing2('LB', 'step')
     call tslb_check(tslb_step(ctx, 1), 'tslb_step')
     call end_timing2('LB', 'step')
-    if (jet .and. mod(step, 10) == 0) then
-      call start_timing2('LB', 'bcs_inflow')
-      call inflow_plane(step)
-      call tslb_check(tslb_set_zplane_bc(ctx, c_loc(w_in), c_loc(phi_in)), 'tslb_set_zplane_bc')
-      call end_timing2('LB', 'bcs_inflow')
-    endif
     if (jet_root .and. mod(step, 10) == 0) then
       ! inflow refresh inside radius 10 (boundary_cds_3D_module.f90:866-877)
       call start_timing2('LB', 'bcs_inflow')
@@ -291,25 +298,43 @@ contains
     enddo
   end subroutine inflow_disk
 
-  ! inflow plane: w = uwall + 0.004 sqrt(-2 ln r) cos(2 pi r) inside the disk,
-  ! with the same r twice as the reference (boundary_cds_3D_module.f90:1253-1255)
-  subroutine inflow_plane(istep)
-    integer, intent(in) :: istep
+  ! newBCS initial inflow: inside the disk w = uwall + 0.02 sqrt(-2 ln ry)
+  ! cos(2 pi rx), rx and ry the generator at (i, j, 1, 524) and (i, j, 1, 1732);
+  ! 0 outside; phi = 1 on the disk (two components)
+  subroutine inflow_initial()
     integer :: ii, jj
-    real :: rr
+    real :: rx, ry
     w_in = 0.0
     phi_in = 0.0
     do jj = 1, ny
       do ii = 1, nx
         if ((real(ii) - nx / 2.0)**2 + (real(jj) - ny / 2.0)**2 <= 10.0**2) then
-          rr = tslb_rand_noseeded(ii, jj, 1, istep)
-          w_in(ii, jj) = uwall + 0.004 * sqrt(-2.0 * log(max(rr, 1.0e-30))) * cos(2.0 * real(pi) * rr)
+          rx = tslb_rand_noseeded(ii, jj, 1, 524)
+          ry = tslb_rand_noseeded(ii, jj, 1, 1732)
+          w_in(ii, jj) = uwall + 0.02 * sqrt(-2.0 * log(max(ry, 1.0e-30))) * cos(2.0 * real(pi) * rx)
           phi_in(ii, jj) = 1.0
         endif
       enddo
     enddo
     w(:, :, 1) = w_in
     phi(:, :, 1) = phi_in
+  end subroutine inflow_initial
+
+  ! inflow redraw: w = uwall + 0.004 sqrt(-2 ln r) cos(2 pi r) inside the disk,
+  ! with the same r twice as the reference (boundary_cds_3D_module.f90:1253-1255);
+  ! outside the disk w keeps its value
+  subroutine inflow_plane(istep)
+    integer, intent(in) :: istep
+    integer :: ii, jj
+    real :: rr
+    do jj = 1, ny
+      do ii = 1, nx
+        if ((real(ii) - nx / 2.0)**2 + (real(jj) - ny / 2.0)**2 <= 10.0**2) then
+          rr = tslb_rand_noseeded(ii, jj, 1, istep)
+          w_in(ii, jj) = uwall + 0.004 * sqrt(-2.0 * log(max(rr, 1.0e-30))) * cos(2.0 * real(pi) * rr)
+        endif
+      enddo
+    enddo
   end subroutine inflow_plane
 
   ! mid-plane slices as stream-unformatted REAL(4), output/out_<f><plane>_NNNNNN.raw

@@ -257,6 +257,9 @@ class Lattice:
             return
         w = np.ascontiguousarray(w_in, dtype=np.float32).reshape(-1)
         ph = None if phi_in is None else np.ascontiguousarray(phi_in, dtype=np.float32).reshape(-1)
+        plane = self.shape[1] * self.shape[2]
+        if w.size != plane or (ph is not None and ph.size != plane):
+            raise ValueError("inflow planes must hold nx*ny values")
         _check(self.lib.tslb_set_zplane_bc(self._h, _fptr(w), None if ph is None else _fptr(ph)),
                "tslb_set_zplane_bc")
 

@@ -155,7 +155,9 @@ int tslb_step(tslb_ctx *ctx, int nsteps);
 int tslb_normals(tslb_ctx *ctx);
 /* K3 fused collide + stream followed by K1 moments of the streamed
  * populations, as one moment-space pass (recursive_2c_TSLB3D.f90:607-836 then
- * :346-542 of the next step); uses the normals of the last tslb_normals(). */
+ * :346-542 of the next step); uses the normals of the last tslb_normals().
+ * The moment-space plane BCs of tslb_set_zplane_bc follow it, as in
+ * tslb_step, so tslb_normals + tslb_collide_stream equals one tslb_step. */
 int tslb_collide_stream(tslb_ctx *ctx);
 
 /* Moment-space z-plane boundary conditions applied after every step:

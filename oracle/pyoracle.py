@@ -51,6 +51,7 @@ def lib():
             getattr(L, f"orc_feq_{sfx}").argtypes = [ct, ct, ct, ct, cp]
             getattr(L, f"orc_aneq_{sfx}").argtypes = [cp, ct, ct, ct, cp]
             getattr(L, f"orc_set_jet_{sfx}").argtypes = [P, ctypes.c_int]
+            getattr(L, f"orc_set_zplane_bc_{sfx}").argtypes = [P, cp, cp]
             getattr(L, f"orc_set_closure_{sfx}").argtypes = [P, cp]
             getattr(L, f"orc_set_colour_{sfx}").argtypes = [P, ctypes.c_double, ctypes.c_double]
             getattr(L, f"orc_colour_values_{sfx}").argtypes = [
@@ -172,6 +173,21 @@ class Oracle:
     def set_jet(self, enable=True):
         """Turbulent-jet inlet (k=0) / outlet (k=nz-1) wall planes; set before init."""
         self._fn("set_jet")(self._h, int(bool(enable)))
+
+    def set_zplane_bc(self, w_in, phi_in=None):
+        """Moment-space z-plane BC (bcs_TSLB_only_z_turbojet): inflow w (ny, nx) of
+        plane k=0, optional phi (two components); applied now and after every
+        step.  None disables it."""
+        if w_in is None:
+            self._fn("set_zplane_bc")(self._h, None, None)
+            return
+        w = np.ascontiguousarray(w_in, dtype=self.dtype).reshape(-1)
+        assert w.size == self.shape[1] * self.shape[2]
+        ph = None
+        if phi_in is not None:
+            ph = np.ascontiguousarray(phi_in, dtype=self.dtype).reshape(-1)
+            assert ph.size == w.size
+        self._fn("set_zplane_bc")(self._h, _ptr(w, self.ct), None if ph is None else _ptr(ph, self.ct))
 
     def set_closure(self, clo):
         """Per-node learned-closure fields (7, nz, ny, nx): nu_t, dsigma; None clears."""

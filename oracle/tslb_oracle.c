@@ -22,6 +22,9 @@
  *   - colour-gradient model    turbulent_benchmark/couette/recursive_colorTSLB3D.f90:
  *                              719-959 (restated with its defects corrected)
  *   - periodic ghost wrap      TSLB_newBCS/boundary_cds_3D_module.f90:1294-1877
+ *   - moment-space z-plane BC  TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292
+ *                              (bcs_TSLB_only_z_turbojet; driver order
+ *                              TSLB_newBCS/recursive_TSbcs_TSLB3.f90:362-771)
  *   - counter-based RNG        TSLB_newBCS/vars_single_3D_module.f90:78-165
  *   - initial state (f=feq, g pre-pushed)  recursive_2c_TSLB3D.f90:106-287
  * and adds a moment-space pull restatement of the same step (orc_pull_step_*)
@@ -97,6 +100,8 @@ typedef struct {
   real *pops;         /* pull scratch: [27][n] */
   real *gpops;        /* pull scratch: [7][n] */
   real *clo;          /* learned closure fields nu_t, dsigma[6]: [7][n] (NULL: none) */
+  real *zw_in;        /* moment-space z-plane BC: inflow w of plane k = 0 (NULL: off) */
+  real *zphi_in;      /* and its phi (NULL: phi copied from k = 1) */
 } ctx_t;
 
 /* ---------------------------------------------------------------------- */
@@ -297,6 +302,8 @@ void FN(orc_destroy)(void *h) {
   free(c->fa);
   free(c->apops);
   free(c->clo);
+  free(c->zw_in);
+  free(c->zphi_in);
   free(c);
 }
 
@@ -318,6 +325,65 @@ void FN(orc_set_closure)(void *h, const real *clo) {
 void FN(orc_set_isfluid)(void *h, const int32_t *isf) {
   ctx_t *c = (ctx_t *)h;
   memcpy(c->isf, isf, c->n * sizeof(int32_t));
+}
+
+/* ---------------------------------------------------------------------- */
+/* Moment-space z-plane BC, bcs_TSLB_only_z_turbojet                         */
+/* (TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292).  The reference calls  */
+/* it after the moments kernel and before the fused collide + push, which    */
+/* runs over every node, boundary planes included                            */
+/* (TSLB_newBCS/recursive_TSbcs_TSLB3.f90:563,617-769).  Its moments kernel  */
+/* skips the two non-fluid planes, so their moments are exactly what this   */
+/* BC writes:                                                                 */
+/*   k = 1  (0 here):     rho, Pi from k = 2; u = v = 0; w = the inflow w    */
+/*                        (:1262-1273; w(i,j,1) is never recomputed, so it    */
+/*                        keeps the last inflow draw, :1253-1255)             */
+/*   k = nz (nz-1 here):  rho, u, v, w, Pi from k = nz-1 (:1275-1285)          */
+/* Here every node is fluid and z is not periodic, so K3/K1 also produce     */
+/* moments on both planes (from bounced-back populations beyond them); the   */
+/* BC overwrites all of them, so the result is the reference's.  Two-         */
+/* component runs (no reference counterpart: the newBCS driver is one-        */
+/* component) take phi of plane 0 from phi_in when given, else from k = 1,   */
+/* and copy phi of plane nz-1 from nz-2.  The inflow redraw every 10 steps   */
+/* (:1237-1259) is the caller's: it passes the new plane here.               */
+/* ---------------------------------------------------------------------- */
+static void zplane_bc(ctx_t *c) {
+  if (!c->zw_in) return;
+  const size_t plane = (size_t)c->nx * c->ny;
+  const size_t lo1 = plane, hi = plane * (size_t)(c->nz - 1), hi1 = plane * (size_t)(c->nz - 2);
+  for (size_t p = 0; p < plane; p++) {
+    for (int q = 0; q < NFIELD; q++) c->m[q][hi + p] = c->m[q][hi1 + p];
+    for (int q = 0; q < NFIELD; q++) {
+      if (q == F_U || q == F_V) c->m[q][p] = 0;
+      else if (q == F_W) c->m[q][p] = c->zw_in[p];
+      else if (q == F_PHI && c->zphi_in) c->m[q][p] = c->zphi_in[p];
+      else c->m[q][p] = c->m[q][lo1 + p];
+    }
+  }
+}
+
+/* Enable the z-plane BC with inflow w_in (nx*ny) and, for two components,
+ * phi_in (or NULL); NULL w_in disables it.  Applied at once to the current
+ * moments, then after every step. */
+void FN(orc_set_zplane_bc)(void *h, const real *w_in, const real *phi_in) {
+  ctx_t *c = (ctx_t *)h;
+  const size_t plane = (size_t)c->nx * c->ny;
+  if (!w_in) {
+    free(c->zw_in);
+    free(c->zphi_in);
+    c->zw_in = c->zphi_in = NULL;
+    return;
+  }
+  if (!c->zw_in) c->zw_in = (real *)malloc(plane * sizeof(real));
+  memcpy(c->zw_in, w_in, plane * sizeof(real));
+  if (phi_in && c->ncomp == 2) {
+    if (!c->zphi_in) c->zphi_in = (real *)malloc(plane * sizeof(real));
+    memcpy(c->zphi_in, phi_in, plane * sizeof(real));
+  } else {
+    free(c->zphi_in);
+    c->zphi_in = NULL;
+  }
+  zplane_bc(c);
 }
 
 /* fields: 11 arrays in the order rho,u,v,w,pxx,pyy,pzz,pxy,pxz,pyz,phi */
@@ -767,6 +833,7 @@ void FN(orc_step)(void *h, int nsteps) {
     if (c->ncomp == 3) resolve(c, c->fa, 27);
     if (c->jet) jet_bc_push(c);
     k1_moments(c);
+    zplane_bc(c);
   }
 }
 
@@ -926,6 +993,7 @@ void FN(orc_pull_step)(void *h, int nsteps) {
       for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) nm[F_W][idx(c, i, j, nz - 1)] = c->m[F_W][idx(c, i, j, nz - 2)];
     for (int q = 0; q < NFIELD; q++) memcpy(c->m[q], nm[q], c->n * sizeof(real));
+    zplane_bc(c);
   }
   for (int q = 0; q < NFIELD; q++) free(nm[q]);
 }

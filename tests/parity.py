@@ -7,6 +7,10 @@ Per-field max relative error, each field normalised by the scale of its group:
   phi                      max|d phi| / max|phi|
 A component that is zero in the reference (w of a 2D-like flow) is then judged
 against the velocity scale of the flow, not against its own round-off.
+
+rho is also judged against its own fluctuation (`drho_error`): the pressure and
+the Laplace jump live in rho - 1, which is ~1e-3 of rho, so a bound relative
+to max|rho| alone would let ~1 % errors in the fluctuation pass.
 """
 import numpy as np
 
@@ -44,3 +48,10 @@ def field_rms_errors(got, ref):
         scale = max(float(np.sqrt(np.mean(ref[list(GROUP[q])] ** 2))), 1e-30)
         out[name] = float(np.sqrt(np.mean((got[q] - ref[q]) ** 2))) / scale
     return out
+
+
+def drho_error(got, ref):
+    """max|d rho| / max|rho_ref - 1| (the density fluctuation's own scale)."""
+    g = np.asarray(got[0], dtype=np.float64)
+    r = np.asarray(ref[0], dtype=np.float64)
+    return float(np.abs(g - r).max()) / max(float(np.abs(r - 1.0).max()), 1e-30)

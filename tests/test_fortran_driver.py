@@ -87,3 +87,44 @@ def test_driver_restart_is_exact_and_vti_matches(tmp_path):
         lat.step(14)
         phi = lat.download_field("phi")
     assert np.array_equal(pa.reshape(n, n, n), phi)
+
+
+@pytest.mark.gpu
+def test_driver_newbcs_jet_matches_python_binding(tmp_path):
+    """config='jet' (the newBCS jet): the inflow redraw of a multiple of 10
+    reaches that step's output and collision, as in the reference's loop
+    (TSLB_newBCS/recursive_TSbcs_TSLB3.f90:563-617).  Frames 1 and 2 (steps 10
+    and 20) equal the Python binding's state at those steps."""
+    nx, ny, nz = 40, 36, 24
+    (tmp_path / "in.nml").write_text(
+        f"&tslb_run\n nx={nx}, ny={ny}, nz={nz}, ncomp=1, nsteps=25, stamp2D=10, tau=0.6,\n"
+        " config='jet', uwall=0.05, noise=0.0\n/\n")
+    res = subprocess.run([DRIVER, "in.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(res.stdout[-1500:], res.stderr[-1500:])
+    assert res.returncode == 0
+    case = configs.jet(nx, ny, nz, tau=0.6)
+    f = case.host_fields()
+    frames = {}
+    with Lattice(nx, ny, nz, **case.lattice) as lat:
+        for name in ("rho", "u", "v", "w", "phi"):
+            lat.upload_field(name, f[name])
+        lat.init_equilibrium(False)
+        wcur = configs.jet_inflow_initial(nx, ny)
+        lat.set_zplane_bc(wcur)
+        for s in range(1, 21):
+            if s % 10 == 0:
+                wcur = configs.jet_inflow_redraw(wcur, s)
+                lat.set_zplane_bc(wcur)
+                frames[s // 10] = lat.download()
+            lat.step(1)
+    # the driver draws the inflow with Fortran's REAL(4) log/sqrt/cos and the
+    # binding with numpy's, which may differ in the last bit; a redraw that
+    # reached the wrong step would move w by ~1e-3
+    for fr, m in frames.items():
+        wxz = np.fromfile(tmp_path / "output" / f"out_wxz_{fr:06d}.raw", dtype=np.float32).reshape(nz, nx)
+        rxy = np.fromfile(tmp_path / "output" / f"out_rhoxy_{fr:06d}.raw", dtype=np.float32).reshape(ny, nx)
+        assert np.abs(wxz - m[3][:, ny // 2 - 1, :]).max() <= 1e-6, fr
+        assert np.abs(rxy - m[0][nz // 2 - 1]).max() <= 1e-6, fr
+    # the inlet row of frame 2 carries the redraw of step 20
+    w20 = configs.jet_inflow_redraw(configs.jet_inflow_redraw(configs.jet_inflow_initial(nx, ny), 10), 20)
+    assert np.array_equal(frames[2][3][0], w20)

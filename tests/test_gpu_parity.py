@@ -23,7 +23,7 @@ import pytest
 import pyoracle as po
 from entropicthreadsafe_amd import configs
 from entropicthreadsafe_amd.tslb import FORCE_ADDITIVE, FORCE_SHIFT, Lattice
-from parity import field_errors
+from parity import drho_error, field_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -60,17 +60,20 @@ def run_oracle(kw, fields, steps, isfluid=None, prepush=True, dtype=np.float64):
     return m
 
 
-def check(kw, fields, steps, isfluid=None, tol64=1e-5, tol32=1e-5):
+def check(kw, fields, steps, isfluid=None, tol64=1e-5, tol32=1e-5, tol_drho=1e-3):
     got = run_gpu(kw, fields, steps, isfluid)
     ref64 = run_oracle(kw, fields, steps, isfluid)
     ref32 = run_oracle(kw, fields, steps, isfluid, dtype=np.float32)
     e64 = field_errors(got, ref64)
     e32 = field_errors(got, ref32)
     eref = field_errors(ref32, ref64)
+    dr, dr_ref = drho_error(got, ref64), drho_error(ref32, ref64)
     print("GPU vs FP64   ", {k: f"{v:.2e}" for k, v in e64.items()})
     print("GPU vs FP32   ", {k: f"{v:.2e}" for k, v in e32.items()})
     print("FP32 vs FP64  ", {k: f"{v:.2e}" for k, v in eref.items()})
+    print(f"rho - 1: GPU vs FP64 {dr:.2e}, FP32 oracle vs FP64 {dr_ref:.2e}")
     assert np.isfinite(got).all()
+    assert dr <= tol_drho, ("drho", dr)
     for f, v in e64.items():
         assert v <= tol64, ("fp64", f, v)
     for f, v in e32.items():
@@ -312,3 +315,41 @@ def test_mass_conservation_long_run():
     assert abs(r1[0] - r0[0]) < 1e-6 * r0[0]
     assert np.all(np.abs(r1[3:6]) < 2e-4)
     assert r1[2] < r0[2]
+
+
+def test_tgv_c1_energy_history_1000_steps():
+    """The Taylor-Green configuration at its size and length (64^3, Re 1600,
+    1000 steps; recursive_TSLB3D.f90:319-711): kinetic energy every 100 steps
+    against the FP64 oracle, and every field at step 1000."""
+    case = configs.tgv(n=64)
+    f = case.host_fields()
+    kw = case.lattice
+    o = po.Oracle(64, 64, 64, **_oracle_kwargs(kw))
+    o.init(*[f[q].astype(np.float64) for q in ("rho", "u", "v", "w", "phi")], False)
+    ke_g, ke_o, mass_g, mass_o = [], [], [], []
+    with Lattice(64, 64, 64, **kw) as lat:
+        for name in ("rho", "u", "v", "w", "phi"):
+            lat.upload_field(name, f[name])
+        lat.init_equilibrium(False)
+        for _ in range(10):
+            lat.step(100)
+            o.step(100)
+            rg, ro = lat.reduce(), o.reduce()
+            ke_g.append(rg[2])
+            ke_o.append(ro[2])
+            mass_g.append(rg[0])
+            mass_o.append(ro[0])
+        got = lat.download()
+    ref = o.moments()
+    o.close()
+    ke_g, ke_o = np.array(ke_g), np.array(ke_o)
+    rel = np.abs(ke_g / ke_o - 1)
+    e = field_errors(got, ref)
+    print("KE GPU", ke_g, "\nKE FP64", ke_o, "\nrelative", rel)
+    print("step 1000 vs FP64", {k: f"{v:.2e}" for k, v in e.items() if k != "phi"},
+          f"drho {drho_error(got, ref):.2e}")
+    assert rel.max() <= 1e-5
+    assert np.all(np.diff(ke_o) < 0) and np.all(np.diff(ke_g) < 0)
+    assert np.abs(np.array(mass_g) / np.array(mass_o) - 1).max() <= 1e-9
+    for k, v in e.items():
+        assert v <= 1e-4, (k, v)

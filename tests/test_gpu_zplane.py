@@ -1,0 +1,106 @@
+"""GPU parity of the moment-space z-plane BC (tslb_set_zplane_bc, replacing
+bcs_TSLB_only_z_turbojet, TSLB_newBCS/boundary_cds_3D_module.f90:1233-1292)
+with the oracle, on the newBCS jet: periodic x and y, inlet plane k = 1 with
+the seeded inflow disk, zero-gradient outlet k = nz, the inflow redrawn every
+10 steps through the reference's generator.
+
+Sequence, as the reference's time loop (TSLB_newBCS/recursive_TSbcs_TSLB3.f90:
+362-771: moments, BC with the redraw when step % 10 == 0, output, collide +
+push): at step s the caller passes the redrawn plane (set_zplane_bc applies
+the BC at once), the state is then the reference's output of step s, and one
+step() collides it.  Both sides get the same host-drawn planes.  Tolerance:
+every field within 1e-5 of the FP64 oracle (tests/parity.py), rho also
+against its own fluctuation max|rho - 1|.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from entropicthreadsafe_amd import configs
+from entropicthreadsafe_amd.tslb import Lattice
+from parity import drho_error, field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(case, nsteps, checkpoints):
+    nz, ny, nx = case.shape
+    kw = case.lattice
+    f = case.host_fields()
+    ncomp = kw["ncomp"]
+    phi_in = f["phi"][0].copy() if ncomp == 2 else None
+    w_in = configs.jet_inflow_initial(nx, ny)
+    got, ref = {}, {}
+    o = po.Oracle(nx, ny, nz, ncomp=ncomp, tau=kw["tau"], sigma=kw.get("sigma", 0.0),
+                  sharp_c=kw.get("sharp_c", 0.0), periodic=kw["periodic"])
+    o.init(*[f[q].astype(np.float64) for q in ("rho", "u", "v", "w", "phi")], case.prepush_phi)
+    with Lattice(nx, ny, nz, **kw) as lat:
+        for name in ("rho", "u", "v", "w", "phi"):
+            lat.upload_field(name, f[name])
+        lat.init_equilibrium(case.prepush_phi)
+        lat.set_zplane_bc(w_in, phi_in)
+        o.set_zplane_bc(w_in, phi_in)
+        wcur = w_in
+        for s in range(1, nsteps + 1):
+            if s % 10 == 0:
+                wcur = configs.jet_inflow_redraw(wcur, s)
+                lat.set_zplane_bc(wcur, phi_in)
+                o.set_zplane_bc(wcur, phi_in)
+            if s in checkpoints:
+                got[s] = lat.download()
+                ref[s] = o.moments()
+            lat.step(1)
+            o.step(1)
+    o.close()
+    return got, ref
+
+
+@pytest.mark.parametrize("tau", [0.6, 0.5004])
+def test_newbcs_jet_1c_matches_oracle(tau):
+    # tau = 0.5004 is the newBCS driver's own value
+    # (TSLB_newBCS/recursive_TSbcs_TSLB3.f90:38)
+    case = configs.jet(nx=40, ny=36, nz=24, tau=tau)
+    got, ref = _run_both(case, 60, (1, 10, 11, 35, 60))
+    for s in sorted(got):
+        e = field_errors(got[s], ref[s])
+        drho = drho_error(got[s], ref[s])
+        print(f"step {s}", {k: f"{v:.2e}" for k, v in e.items() if k != "phi"}, f"drho {drho:.2e}")
+        assert np.isfinite(got[s]).all()
+        for k, v in e.items():
+            assert v <= 1e-5, (s, k, v)
+        assert drho <= 1e-3, (s, drho)
+    # the redrawn inflow is on the inlet plane at the redraw step
+    w10 = configs.jet_inflow_redraw(configs.jet_inflow_initial(40, 36), 10)
+    assert np.array_equal(got[10][3][0], w10)
+
+
+def test_newbcs_jet_2c_matches_oracle():
+    case = configs.jet(nx=40, ny=36, nz=24, tau=0.8, ncomp=2, sigma=0.01)
+    got, ref = _run_both(case, 25, (5, 15, 25))
+    for s in sorted(got):
+        e = field_errors(got[s], ref[s])
+        print(f"step {s}", {k: f"{v:.2e}" for k, v in e.items()})
+        for k, v in e.items():
+            assert v <= 1e-5, (s, k, v)
+
+
+def test_split_entry_points_equal_step_with_zplane_bc():
+    nx, ny, nz = 24, 20, 16
+    case = configs.jet(nx, ny, nz, tau=0.7)
+    f = case.host_fields()
+    w_in = configs.jet_inflow_initial(nx, ny)
+    outs = []
+    for split in (False, True):
+        with Lattice(nx, ny, nz, **case.lattice) as lat:
+            for name in ("rho", "u", "v", "w", "phi"):
+                lat.upload_field(name, f[name])
+            lat.init_equilibrium(False)
+            lat.set_zplane_bc(w_in)
+            for _ in range(6):
+                if split:
+                    lat.normals()
+                    lat.collide_stream()
+                else:
+                    lat.step(1)
+            outs.append(lat.download())
+    assert np.array_equal(outs[0], outs[1])
