@@ -37,13 +37,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Cores this process may use: its CPU affinity, capped by OMP_NUM_THREADS
+    when the host sets it (a shared GPU host allots each GPU its share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(n, 1)
+
+
 def cpu_baseline(n, case, budget_s=15.0, threads=None):
     """Time the FP32 CPU oracle (population push, as the reference) on a slab
-    n x n x nzs of the emulsion workload, periodic."""
+    n x n x nzs of the emulsion workload, periodic, on every core this process
+    may use."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_cores()
     po.set_threads(threads)
     nzs = max(4, min(32, n // 16))
     f = {k: np.ascontiguousarray(v[:nzs]) for k, v in case.fields.items()}
@@ -66,8 +80,10 @@ def cpu_baseline(n, case, budget_s=15.0, threads=None):
     o.close()
     mlups = n * n * nzs * steps / dt / 1e6
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(),
             "sample": f"FP32 population-push oracle, emulsion slab {n}x{n}x{nzs} periodic, "
-                      f"{steps} steps in {dt:.1f} s"}
+                      f"{steps} steps in {dt:.1f} s on {threads} threads (all cores allotted to "
+                      f"this process; the host has {os.cpu_count()})"}
 
 
 def load_traffic(path):
@@ -95,6 +111,10 @@ def main():
                     help="moment storage precision (compute is FP32)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the emulsion line at ~15 %% volume fraction")
+    ap.add_argument("--no-les", action="store_true",
+                    help="skip the closure-LES figures of the default line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1 (the multi-GPU slab "
@@ -121,36 +141,16 @@ def main():
     nz, ny, nx = case.shape
     ncomp = case.lattice["ncomp"]
     log(f"bench: {args.workload} {nx}x{ny}x{nz} set up in {time.time() - t0:.1f} s ({case.notes})")
+    m = measure(Lattice, case, args.storage, args.steps, args.warmup, copy=True)
     nodes = nx * ny * nz
-    with Lattice(nx, ny, nz, device=0, storage=args.storage, **case.lattice) as lat:
-        case.apply(lat)
-        log(f"bench: state on device ({lat.device_bytes() / 2**30:.2f} GiB)")
-        copy_gbs = lat.stream_copy_gbs()
-        log(f"bench: plain copy kernel {copy_gbs:.0f} GB/s")
-        lat.step(args.warmup)
-        lat.sync()
-        r0 = lat.reduce()
-        lat.timing_enable(True)
-        lat.sync()
-        t0 = time.perf_counter()
-        lat.step(args.steps)
-        lat.sync()
-        wall = time.perf_counter() - t0
-        fused_ms, fused_n = lat.timing_read(0)
-        norm_ms, norm_n = lat.timing_read(1)
-        lat.timing_enable(False)
-        r1 = lat.reduce()
-    log(f"bench: {args.steps} steps in {wall:.3f} s; fused {fused_ms / max(fused_n, 1):.3f} ms x {fused_n}, "
-        f"normals {norm_ms / max(norm_n, 1):.3f} ms x {norm_n}; mass drift {(r1[0] - r0[0]) / r0[0]:.2e}, "
-        f"phi drift {(r1[1] - r0[1]) / max(r0[1], 1e-30):.2e}")
 
-    mlups = nodes * args.steps / wall / 1e6
+    mlups = nodes * args.steps / m["wall"] / 1e6
     nfields = 11 if ncomp >= 2 else 10
     es = 4 if args.storage == "fp32" else 2
     bytes_per_launch = 2.0 * es * nfields * nodes     # read + write of the stored moment fields
-    t_fused = fused_ms / max(fused_n, 1) / 1e3
+    t_fused = m["fused_ms"] / 1e3
     achieved = bytes_per_launch / t_fused / 1e9
-    t_step = (fused_ms + norm_ms) / max(fused_n, 1) / 1e3
+    t_step = m["fused_ms"] / 1e3 + m["curv_ms"] / 1e3
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
     # with the copy kernel), for the default size and FP32 storage only
     traffic = None
@@ -163,7 +163,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(m["wall"] / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -174,19 +174,103 @@ def main():
                    **{k: v for k, v in case.lattice.items() if k in ("tau", "sigma", "sharp_c", "lin_force", "force")}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("stored figure: rocprofv3 FETCH_SIZE (x2 copy-kernel calibration) + "
+                                        "WRITE_SIZE of this kernel at this size, "
+                                        f"profiles/fused_{args.workload}_traffic.json; not measured in this run")
+                     if traffic else None,
                      "kernel": f"k_fused<{ncomp}>", "algorithmic_bytes_per_launch": bytes_per_launch,
                      "mean_launch_ms": round(t_fused * 1e3, 4)},
         "step_roofline_frac": round(bytes_per_launch / t_step / 1e9 / HBM_PEAK_GBS, 4),
-        "hbm_stream_copy_gbs": round(copy_gbs, 1),
-        "fused_frac_of_copy": round(achieved / copy_gbs, 4),
+        "hbm_stream_copy_gbs": round(m["copy_gbs"], 1),
+        "fused_frac_of_copy": round(achieved / m["copy_gbs"], 4),
+        "curvature_ms_per_step": round(m["curv_ms"], 4),
     }
+    if args.workload == "emulsion" and n == 512 and not args.no_dense:
+        # the same workload at the ~15 % volume fraction its description
+        # names: 2900 droplets of radii U[8, 16] (200 droplets fill 1.2 %)
+        t0 = time.time()
+        dense = configs.emulsion(n=n, ndrop=2900)
+        log(f"bench: dense emulsion set up in {time.time() - t0:.1f} s ({dense.notes})")
+        dsteps = max(20, args.steps // 2)
+        d = measure(Lattice, dense, args.storage, dsteps, args.warmup)
+        out["emulsion_dense"] = {
+            "notes": dense.notes, "steps": dsteps,
+            "mlups": round(nodes * dsteps / d["wall"] / 1e6, 2),
+            "ms_per_step": round(d["wall"] / dsteps * 1e3, 4),
+            "fused_ms": round(d["fused_ms"], 4), "curvature_ms": round(d["curv_ms"], 4),
+            "curvature_share": round(d["curv_ms"] / (d["fused_ms"] + d["curv_ms"]), 4),
+            "fused_roofline_frac": round(bytes_per_launch / (d["fused_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "step_roofline_frac": round(bytes_per_launch / ((d["fused_ms"] + d["curv_ms"]) / 1e3) / 1e9
+                                        / HBM_PEAK_GBS, 4),
+        }
     if not args.no_cpu and args.workload == "emulsion" and args.storage == "fp32":
         try:
             out["cpu_baseline"] = cpu_baseline(n, case, args.cpu_budget)
         except Exception as e:  # the baseline is reported, never the product
             log(f"bench: cpu baseline failed: {e}")
             out["cpu_baseline"] = None
+    if args.workload == "emulsion" and n == 512 and not args.no_les:
+        # the second half of the metric: the closure LES against the plain
+        # coarse run and the filtered fine run (entropicthreadsafe_amd/les.py)
+        try:
+            out["closure_les"] = closure_les_summary(les_run(log))
+        except Exception as e:
+            log(f"bench: closure LES failed: {e}")
+            out["closure_les"] = None
     print(json.dumps(out), flush=True)
+
+
+def measure(Lattice, case, storage, steps, warmup, copy=False):
+    """Time `steps` steps of a case after `warmup`, inputs resident in HBM:
+    wall time, mean fused-kernel and curvature-pass launch times (HIP events
+    on the library's stream), the copy-kernel rate (copy=True) and the
+    mass / phase-mass drift over the timed steps."""
+    nz, ny, nx = case.shape
+    with Lattice(nx, ny, nz, device=0, storage=storage, **case.lattice) as lat:
+        case.apply(lat)
+        log(f"bench: state on device ({lat.device_bytes() / 2**30:.2f} GiB)")
+        copy_gbs = None
+        if copy:
+            copy_gbs = lat.stream_copy_gbs()
+            log(f"bench: plain copy kernel {copy_gbs:.0f} GB/s")
+        lat.step(warmup)
+        lat.sync()
+        r0 = lat.reduce()
+        lat.timing_enable(True)
+        lat.sync()
+        t0 = time.perf_counter()
+        lat.step(steps)
+        lat.sync()
+        wall = time.perf_counter() - t0
+        fused_ms, fused_n = lat.timing_read(0)
+        norm_ms, norm_n = lat.timing_read(1)
+        lat.timing_enable(False)
+        r1 = lat.reduce()
+    fused = fused_ms / max(fused_n, 1)
+    curv = norm_ms / max(fused_n, 1)       # per step (the pass runs once per step)
+    log(f"bench: {steps} steps in {wall:.3f} s; fused {fused:.3f} ms x {fused_n}, "
+        f"curvature {norm_ms / max(norm_n, 1):.3f} ms x {norm_n}; mass drift {(r1[0] - r0[0]) / r0[0]:.2e}, "
+        f"phi drift {(r1[1] - r0[1]) / max(r0[1], 1e-30):.2e}")
+    return dict(wall=wall, fused_ms=fused, curv_ms=curv, copy_gbs=copy_gbs,
+                mass_drift=(r1[0] - r0[0]) / r0[0], phase_drift=(r1[1] - r0[1]) / max(r0[1], 1e-30))
+
+
+def les_run(log_fn, n=512):
+    from entropicthreadsafe_amd import les
+    return les.run(n=n, log=log_fn)
+
+
+def closure_les_summary(res):
+    return {
+        "speedup_closure_vs_fine": round(res["speedup_closure_vs_fine"], 2),
+        "closure_spectrum_error": res["closure_spectrum_error"],
+        "plain_spectrum_error": res["plain_spectrum_error"],
+        "kmax": res["kmax"],
+        "fine_seconds": round(res["fine_seconds"], 4),
+        "closure_seconds": round(res["closure_seconds"], 4),
+        "plain_seconds": round(res["plain_seconds"], 4),
+        "workload": res["workload"],
+    }
 
 
 def closure_les(args):

@@ -72,8 +72,17 @@ __host__ __device__ constexpr int dir_of(int cx, int cy, int cz) {
   return r;
 }
 
-// LDS slot of a direction that moves in the x-y plane (all but 0, 5, 6)
-__host__ __device__ constexpr int qslot(int l) { return l <= 4 ? l - 1 : l - 3; }
+// LDS slot of a direction that moves in the x-y plane (all but 0, 5, 6):
+// 8 (c_z + 1) + d(c_x, c_y), d the in-plane direction index (row-major over
+// c_y, c_x with (0,0) left out), so the three populations of one in-plane
+// direction sit 8 slots apart and a ring source writes its entering ones
+// through three per-lane addresses with constant offsets
+__host__ __device__ constexpr int dslot(int cx, int cy) {
+  return (cy + 1) * 3 + (cx + 1) - ((cy + 1) * 3 + (cx + 1) > 4 ? 1 : 0);
+}
+__host__ __device__ constexpr int qslot(int l) {
+  return 8 * (kCZ[l] + 1) + dslot(kCX[l], kCY[l]);
+}
 
 // 27-bit masks of the directions whose source y - c_l lies beyond a face
 __host__ __device__ constexpr uint32_t dirmask(int axis, int sign) {
@@ -249,19 +258,22 @@ struct ArraySink {
 // Each M_abc is stored pre-multiplied by 2^-(number of nonzero exponents) so
 // that the 1D inverse (m0, m1, m2) -> f(-1) = m2 - m1, f(0) = m0 - 2 m2,
 // f(+1) = m2 + m1 costs three operations per triple.
-template <bool ADD, typename Sink>
-__device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, float Vz,
-                                            const Params &p, Sink &h, float om1) {
-  const float drho = m[0], rho = 1.f + drho;
-  const float ux = m[1], uy = m[2], uz = m[3];
-  const float Bxx = om1 * m[4], Byy = om1 * m[5], Bzz = om1 * m[6];
-  const float Bxy = om1 * m[7], Bxz = om1 * m[8], Byz = om1 * m[9];
+// The raw moments M[a][b][c] (scaled as above) of one source from its
+// moments (rho - 1, u, Pi), the equilibrium velocity V and the additive force.
+template <bool ADD>
+__device__ __forceinline__ void raw_moments(float drho, float ux, float uy, float uz, float Pxx, float Pyy,
+                                            float Pzz, float Pxy, float Pxz, float Pyz, float Vx, float Vy,
+                                            float Vz, float fx, float fy, float fz, float om1,
+                                            float (&M)[3][3][3]) {
+  const float rho = 1.f + drho;
+  const float Bxx = om1 * Pxx, Byy = om1 * Pyy, Bzz = om1 * Pzz;
+  const float Bxy = om1 * Pxy, Bxz = om1 * Pxz, Byz = om1 * Pyz;
   const float rVx = rho * Vx, rVy = rho * Vy, rVz = rho * Vz;
   float a1x = rVx, a1y = rVy, a1z = rVz;
   if constexpr (ADD) {
-    a1x += p.fx;
-    a1y += p.fy;
-    a1z += p.fz;
+    a1x += fx;
+    a1y += fy;
+    a1z += fz;
   }
   const float rVxy = rVx * Vy, rVxz = rVx * Vz, rVyz = rVy * Vz;
   const float a2xx = fmaf(rVx, Vx, Bxx), a2yy = fmaf(rVy, Vy, Byy), a2zz = fmaf(rVz, Vz, Bzz);
@@ -276,7 +288,6 @@ __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, 
   const float a3xyz = fmaf(rVxy, Vz, fmaf(ux, Byz, fmaf(uy, Bxz, uz * Bxy)));
   constexpr float c3 = 1.f / 3.f, c9 = 1.f / 9.f;
   // M[a][b][c], scaled by 2^-(#nonzero)
-  float M[3][3][3];
   M[0][0][0] = drho;
   M[1][0][0] = 0.5f * a1x;
   M[0][1][0] = 0.5f * a1y;
@@ -310,6 +321,14 @@ __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, 
   M[2][1][2] = fmaf(0.5f * c3, M[2][1][0] + M[0][1][2], -(0.25f * c9) * M[0][1][0]);
   M[1][2][2] = fmaf(0.5f * c3, M[1][2][0] + M[1][0][2], -(0.25f * c9) * M[1][0][0]);
   M[2][2][2] = fmaf(0.25f * c9, s200_020 + M[0][0][2], drho * (-1.f / 108.f));
+}
+
+template <bool ADD, typename Sink>
+__device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, float Vz,
+                                            const Params &p, Sink &h, float om1) {
+  float M[3][3][3];
+  raw_moments<ADD>(m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9], Vx, Vy, Vz, p.fx, p.fy, p.fz,
+                   om1, M);
   // pass z: G[a][b][cz]
   float G[3][3][3];
 #pragma unroll
@@ -347,6 +366,52 @@ __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, 
 #undef MT_ROW
 }
 
+
+// A ring source of k_fused (one node outside the tile's 64x8 columns) sends
+// only the 9 populations with c_P = sgn into the tile, P the axis normal to
+// its side of the ring (y for the rows above and below the tile, x for the
+// columns beside it).  Those 9 take the P pass at c_P = sgn first (one
+// operation per (a, c) pair), then the z pass and the in-plane pass along the
+// other axis Q on three triples: 27 operations instead of the 81 of all 27.
+// Columns run the same arithmetic with x and y exchanged in the inputs, so
+// the M[a][b][c] below carry Q, P, z exponents.  rq[q + 1] addresses the LDS
+// slot of the c_z = -1 population moving q along Q; c_z = 0 and +1 sit 8 and
+// 16 slots further (qslot).
+template <bool ADD>
+__device__ __forceinline__ void ring_pops(const float *m, float Vx, float Vy, float Vz, const Params &p,
+                                          float om1, bool col, float sgn, float *const (&rq)[3]) {
+  const float uq = col ? m[2] : m[1], up = col ? m[1] : m[2];
+  const float Pqq = col ? m[5] : m[4], Ppp = col ? m[4] : m[5];
+  const float Pqz = col ? m[9] : m[8], Ppz = col ? m[8] : m[9];
+  const float Vq = col ? Vy : Vx, Vp = col ? Vx : Vy;
+  float fq = 0.f, fp = 0.f;
+  if constexpr (ADD) {
+    fq = col ? p.fy : p.fx;
+    fp = col ? p.fx : p.fy;
+  }
+  float M[3][3][3];
+  raw_moments<ADD>(m[0], uq, up, m[3], Pqq, Ppp, m[6], m[7], Pqz, Ppz, Vq, Vp, Vz, fq, fp, p.fz, om1, M);
+  float K[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) K[a][c] = fmaf(sgn, M[a][1][c], M[a][2][c]);
+  float G[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float m0 = K[a][0], m1 = K[a][1], m2 = K[a][2];
+    G[a][0] = m2 - m1;
+    G[a][1] = fmaf(-2.f, m2, m0);
+    G[a][2] = m2 + m1;
+  }
+#pragma unroll
+  for (int cz = 0; cz < 3; cz++) {
+    const float m0 = G[0][cz], m1 = G[1][cz], m2 = G[2][cz];
+    rq[0][cz * 8 * NSLOT] = m2 - m1;
+    rq[1][cz * 8 * NSLOT] = fmaf(-2.f, m2, m0);
+    rq[2][cz * 8 * NSLOT] = m2 + m1;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Colour-gradient stencil, grouped as the reference (recursive_2c_TSLB3D.f90:
@@ -701,6 +766,38 @@ __device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p,
   }
 }
 
+// A ring source (one- and two-component models): eval_src restricted to the
+// 9 populations entering the tile (ring_pops) and, for the phase field, the
+// flux A_P along the side's normal axis (rf: LDS slot 24 for A_x, 25 for A_y).
+template <int NC, bool ADD, bool CLO>
+__device__ __forceinline__ void eval_ring(const Src<NC, CLO> &s, const Params &p, bool col, float sgn,
+                                          float *const (&rq)[3], float *rf) {
+  static_assert(NC == 1 || NC == 2, "the colour model's ring keeps the full evaluation");
+  const float *m = s.m;
+  float cbuf[10], om1;
+  const float *mc = closure_moments(s, p, cbuf, om1);
+  if constexpr (NC == 2) {
+    const float phi = m[10];
+    const float n0 = s.nn[0], n1 = s.nn[1], n2 = s.nn[2];
+    const float curv = s.kap;
+    const float sx = p.sigma * curv * n0, sy = p.sigma * curv * n1, sz = p.sigma * curv * n2;
+    const float esx = sx + p.fx + p.lin_a * m[1];
+    const float esy = sy + p.fy + p.lin_a * m[2];
+    const float esz = sz + p.fz + p.lin_a * m[3];
+    ring_pops<false>(mc, m[1] + esx, m[2] + esy, m[3] + esz, p, om1, col, sgn, rq);
+    const float sh = 0.125f * p.sharp_c * phi * (1.f - phi);
+    const float base = 0.125f * phi;
+    *rf = fmaf(3.f * base, col ? m[1] : m[2], sh * (col ? n0 : n1));
+  } else {
+    if (p.shift) {
+      ring_pops<false>(mc, m[1] + (p.fx + p.lin_a * m[1]), m[2] + (p.fy + p.lin_a * m[2]),
+                       m[3] + (p.fz + p.lin_a * m[3]), p, om1, col, sgn, rq);
+    } else {
+      ring_pops<ADD>(mc, m[1], m[2], m[3], p, om1, col, sgn, rq);
+    }
+  }
+}
+
 // moment sums of shifted populations for one destination plane
 struct Acc {
   float s0, sx, sy, sz, sxx, syy, szz, sxy, sxz, syz, phi;
@@ -842,7 +939,7 @@ __device__ __forceinline__ int run_len(uint64_t all, int k) {
   return min(len, kRunMax - (k & (kRunMax - 1)));
 }
 
-template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false>
+template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false, bool R9 = true>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_fused(
     const S *__restrict__ src, S *__restrict__ dst, const float *__restrict__ nrm,
     const uint8_t *__restrict__ flags, const uint32_t *__restrict__ links, const float *__restrict__ clo,
@@ -1066,10 +1163,20 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         cr.nn[2] = nnr[2];
         cr.m[NF - 1] = p0[cr_c];
       }
-      PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
-      eval_src<NC, ADD>(cr, p, sk, gq);
-      if constexpr (NC == 3)
-        for (int k = 0; k < 5; k++) rc[k] = gq[k];
+      if constexpr (R9 && NC <= 2) {
+        // the 9 populations (and the phase flux) that enter the tile
+        const bool col = tid >= 2 * SW;
+        const float sgn = (tid < SW || (tid >= 2 * SW && tid < 2 * SW + TY)) ? 1.f : -1.f;
+        const int sg = sgn > 0.f ? 1 : -1, rs = rsy * SW + rsx;
+        float *const rq[3] = {&lds[col ? dslot(sg, -1) : dslot(-1, sg)][rs], &lds[col ? dslot(sg, 0) : dslot(0, sg)][rs],
+                              &lds[col ? dslot(sg, 1) : dslot(1, sg)][rs]};
+        eval_ring<NC, ADD>(cr, p, col, sgn, rq, &lds[col ? 24 : 25][rs]);
+      } else {
+        PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
+        eval_src<NC, ADD>(cr, p, sk, gq);
+        if constexpr (NC == 3)
+          for (int k = 0; k < 5; k++) rc[k] = gq[k];
+      }
     }
     __builtin_amdgcn_s_setprio(0);
     __syncthreads();
@@ -1166,9 +1273,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         }
       } else {
         const float rho = 1.f + aA.s0;
-        // reciprocal with one Newton step (about 0.5 ulp)
+        // 1/rho: v_rcp_f32 refined by one Newton step in residual form.  The
+        // residual 1 - rho inv is exact in the fma, so the step keeps all of
+        // it and gives the correctly rounded reciprocal in nearly every case.
+        // (The product form inv (2 - rho inv) rounds 2 - rho inv next to 1,
+        // which biases 1/rho low: u then lost ~1.7e-8 of itself per step, a
+        // kinetic-energy drift of 3.4e-5 over 1000 Taylor-Green steps.)
         float inv = __builtin_amdgcn_rcpf(rho);
-        inv = inv * fmaf(-rho, inv, 2.f);
+        inv = fmaf(inv, fmaf(-rho, inv, 1.f), inv);
         const float u = aA.sx * inv, v = aA.sy * inv, w = aA.sz * inv;
         const float t = aA.s0 * (1.f / 3.f);
         const float out[11] = {aA.s0,
@@ -2136,7 +2248,8 @@ struct tslb_ctx {
   int zbc = 0, zbc_phi = 0;
   int jet = 0;     // turbulent-jet inlet/outlet planes (k_jet_bc after k_fused)
   int zwalls = 0;  // bit 0: plane 0 all solid; bit 1: plane nz-1 all solid
-  int variant = 2;  // bit 1: cap VGPRs at 128 (two blocks per CU)
+  int variant = 2;  // bit 1: cap VGPRs at 128 (two blocks per CU); bit 2: for comparisons, the
+                    // phase-field FP32 kernel with the full 27-population ring evaluation
   // learned closure: 0 none, 1 fixed per-node fields, 2 recomputed before
   // every collision by `model` from the current state
   int clo_mode = 0;
@@ -2298,8 +2411,19 @@ int launch_fused(tslb_ctx *c) {
         hipLaunchKernelGGL((k_fused<NC, MASK, ADD, 4, S, true>), dim3(nblk), dim3(NTHR), 0,        \
                            c->stream, s, d, c->nrm, c->flags, c->links, c->clo, ilist, c->flags_alt, p); \
     } else {                                                                                       \
-      hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream, s, \
-                         d, c->nrm, c->flags, c->links, (const float *)nullptr, ilist, c->flags_alt, p); \
+      bool done = false;                                                                           \
+      if constexpr (NC == 2 && !MASK && WPE == 4 && sizeof(S) == 4) {                               \
+        if (c->variant & 4) {                                                                      \
+          hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S, false, false>), dim3(nblk), dim3(NTHR), \
+                             0, c->stream, s, d, c->nrm, c->flags, c->links, (const float *)nullptr, \
+                             ilist, c->flags_alt, p);                                              \
+          done = true;                                                                             \
+        }                                                                                          \
+      }                                                                                            \
+      if (!done)                                                                                   \
+        hipLaunchKernelGGL((k_fused<NC, MASK, ADD, WPE, S>), dim3(nblk), dim3(NTHR), 0, c->stream,  \
+                           s, d, c->nrm, c->flags, c->links, (const float *)nullptr, ilist,        \
+                           c->flags_alt, p);                                                       \
     }                                                                                              \
   } while (0)
     // FP32 storage keeps both register budgets selectable for sweeps (capped
@@ -2513,7 +2637,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     k.isc[q] = 1.f / sc;
   }
 
-  if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 3;
+  if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 7;
   int rc = TSLB_OK;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {

@@ -52,6 +52,7 @@ def lib():
             getattr(L, f"orc_aneq_{sfx}").argtypes = [cp, ct, ct, ct, cp]
             getattr(L, f"orc_set_jet_{sfx}").argtypes = [P, ctypes.c_int]
             getattr(L, f"orc_set_zplane_bc_{sfx}").argtypes = [P, cp, cp]
+            getattr(L, f"orc_curvature_{sfx}").argtypes = [P, cp]
             getattr(L, f"orc_set_closure_{sfx}").argtypes = [P, cp]
             getattr(L, f"orc_set_colour_{sfx}").argtypes = [P, ctypes.c_double, ctypes.c_double]
             getattr(L, f"orc_colour_values_{sfx}").argtypes = [
@@ -222,6 +223,13 @@ class Oracle:
     def normals(self):
         out = np.zeros((3,) + self.shape, dtype=self.dtype)
         self._fn("get_normals")(self._h, _ptr(out, self.ct))
+        return out
+
+    def curvature(self):
+        """Normals of the current phi, then the gated curvature kappa = -div(n)
+        of every node (0 where phi(1-phi)|n|^2 <= 0.2); also refreshes normals()."""
+        out = np.zeros(self.shape, dtype=self.dtype)
+        self._fn("curvature")(self._h, _ptr(out, self.ct))
         return out
 
     def reduce(self):

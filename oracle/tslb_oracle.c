@@ -518,8 +518,9 @@ static void k2_normals(ctx_t *c) {
       }
 }
 
-/* CSF shift sigma*kappa*n at fluid node p, recursive_2c_TSLB3D.f90:614-636 */
-static void csf_shift(const ctx_t *c, int i, int j, int k, real *s3) {
+/* Gated curvature at fluid node p, recursive_2c_TSLB3D.f90:614-631:
+ * kappa = -div(n) where phi(1-phi)|n|^2 > 0.2, else 0 */
+static real node_curv(const ctx_t *c, int i, int j, int k) {
   size_t p = idx(c, i, j, k);
   real phi = c->m[F_PHI][p];
   real n0 = c->nrm[0][p], n1 = c->nrm[1][p], n2 = c->nrm[2][p];
@@ -532,9 +533,31 @@ static void csf_shift(const ctx_t *c, int i, int j, int k, real *s3) {
     grad_stencil(c, i, j, k, 2, dz);
     curv = -(dx[0] + dy[1] + dz[2]);
   }
+  return curv;
+}
+
+/* CSF shift sigma*kappa*n at fluid node p, recursive_2c_TSLB3D.f90:614-636 */
+static void csf_shift(const ctx_t *c, int i, int j, int k, real *s3) {
+  size_t p = idx(c, i, j, k);
+  real n0 = c->nrm[0][p], n1 = c->nrm[1][p], n2 = c->nrm[2][p];
+  real curv = node_curv(c, i, j, k);
   s3[0] = c->sigma * curv * n0;
   s3[1] = c->sigma * curv * n1;
   s3[2] = c->sigma * curv * n2;
+}
+
+/* K2 on the current phi, then the gated curvature of every fluid node (0
+ * on solids and where the gate is closed) into out (n values). */
+void FN(orc_curvature)(void *h, real *out) {
+  ctx_t *c = (ctx_t *)h;
+  k2_normals(c);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < c->nz; k++)
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        size_t p = idx(c, i, j, k);
+        out[p] = c->isf[p] ? node_curv(c, i, j, k) : (real)0;
+      }
 }
 
 /* ---------------------------------------------------------------------- */

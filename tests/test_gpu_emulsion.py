@@ -23,12 +23,20 @@ from parity import drho_error, field_errors, field_rms_errors
 
 pytestmark = pytest.mark.gpu
 
-# reduced storage against the FP64 oracle after 15 steps, max relative error
-# per field group (about 1.5x the errors measured on the MI355X)
+# Reduced storage against the FP64 oracle after 15 steps: max and RMS error
+# per field group (tests/parity.py scales), about 1.5x the values measured on
+# the MI355X.  The max is set by the curvature gate phi(1-phi)|n|^2 > 0.2
+# (recursive_2c_TSLB3D.f90:615-616): 16-bit phi moves a few interface nodes
+# across it, and a node that gains or loses its CSF shift sigma*kappa*n
+# (~3e-3 here, against |u| ~ 1e-2) differs by tens of percent of the velocity
+# scale while the RMS stays at the storage rounding.
 TOL_SMALL = {
-    "fp16": dict(rho=2e-4, u=3e-3, p=3e-2, phi=3e-3),
-    "bf16": dict(rho=2e-3, u=3e-2, p=0.3, phi=2e-2),
+    "fp16": dict(max=dict(rho=1.3e-3, u=0.23, p=0.29, phi=7e-3),
+                 rms=dict(rho=1.4e-5, u=9e-3, p=1.3e-2, phi=8e-4)),
+    "bf16": dict(max=dict(rho=1.6e-3, u=0.31, p=0.43, phi=0.105),
+                 rms=dict(rho=4e-5, u=2.6e-2, p=3.5e-2, phi=1e-2)),
 }
+GROUPS = dict(rho=("rho",), u=("u", "v", "w"), p=("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"), phi=("phi",))
 
 
 def small_emulsion():
@@ -68,7 +76,7 @@ def test_small_emulsion_fp32_matches_fp64_oracle():
     assert (phi * (1 - phi) > 0.2).sum() > 500
     for k, v in e.items():
         assert v <= 1e-5, (k, v)
-    assert dr <= 1e-3
+    assert dr <= 1e-5
     assert abs(r1[0] - r0[0]) <= 1e-9 * r0[0]
     assert abs(r1[1] - r0[1]) <= 1e-7 * r0[1]
 
@@ -82,17 +90,19 @@ def test_small_emulsion_reduced_storage(storage):
     print(f"emulsion 64^3 {storage} vs FP64 max", {k: f"{v:.2e}" for k, v in e.items()})
     print(f"emulsion 64^3 {storage} vs FP64 rms", {k: f"{v:.2e}" for k, v in rms.items()})
     print(f"mass drift {(r1[0] - r0[0]) / r0[0]:.2e} phase mass drift {(r1[1] - r0[1]) / r0[1]:.2e}")
-    tol = TOL_SMALL[storage]
     assert np.isfinite(got).all()
-    assert e["rho"] <= tol["rho"]
-    assert max(e["u"], e["v"], e["w"]) <= tol["u"]
-    assert max(e[k] for k in ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")) <= tol["p"]
-    assert e["phi"] <= tol["phi"]
+    for kind, errs in (("max", e), ("rms", rms)):
+        for g, names in GROUPS.items():
+            worst = max(errs[k] for k in names)
+            assert worst <= TOL_SMALL[storage][kind][g], (storage, kind, g, worst)
 
 
+# 200 steps at 512^3 measured on the MI355X: FP32 mass -1.6e-12, phase mass
+# 1.1e-10; BF16 storage mass -2.2e-9, phase mass -1.5e-2 (8 significant bits
+# of phi: the rounding of phi ~ 1 - 1e-3 in the bulk drifts the phase mass)
 @pytest.mark.parametrize("storage,mass_tol,phase_tol", [
-    ("fp32", 1e-9, 1e-6),
-    ("bf16", 1e-4, 3e-2),
+    ("fp32", 1e-9, 1e-8),
+    ("bf16", 1e-8, 2.3e-2),
 ])
 def test_emulsion_512_long_run(storage, mass_tol, phase_tol):
     case = configs.emulsion()  # 512^3, 200 droplets, radii U[8, 16]

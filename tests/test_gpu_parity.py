@@ -60,7 +60,7 @@ def run_oracle(kw, fields, steps, isfluid=None, prepush=True, dtype=np.float64):
     return m
 
 
-def check(kw, fields, steps, isfluid=None, tol64=1e-5, tol32=1e-5, tol_drho=1e-3):
+def check(kw, fields, steps, isfluid=None, tol64=1e-5, tol32=1e-5, tol_drho=1e-4):
     got = run_gpu(kw, fields, steps, isfluid)
     ref64 = run_oracle(kw, fields, steps, isfluid)
     ref32 = run_oracle(kw, fields, steps, isfluid, dtype=np.float32)
@@ -95,7 +95,9 @@ def test_tgv_c1_size_64():
 def test_channel_walls_additive_force_ragged():
     # ragged extents (not multiples of the 64x8 tile); tau = 0.536
     case = configs.channel(nx=20, ny=12, nz=10, re_tau=5.0, u_tau=0.005, pert=0.05)
-    check(case.lattice, case.host_fields(), 150)
+    # rho - 1 within 2e-4 of its fluctuation scale (measured 1.3e-4; the
+    # reference's FP32 arithmetic: 1.2e-3)
+    check(case.lattice, case.host_fields(), 150, tol_drho=2e-4)
 
 
 def test_fp32_closer_to_fp64_than_reference_fp32_near_half_tau():
@@ -352,4 +354,4 @@ def test_tgv_c1_energy_history_1000_steps():
     assert np.all(np.diff(ke_o) < 0) and np.all(np.diff(ke_g) < 0)
     assert np.abs(np.array(mass_g) / np.array(mass_o) - 1).max() <= 1e-9
     for k, v in e.items():
-        assert v <= 1e-4, (k, v)
+        assert v <= 1e-5, (k, v)
