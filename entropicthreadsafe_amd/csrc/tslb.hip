@@ -39,6 +39,7 @@
 #include <sys/resource.h>
 #include <sys/sysinfo.h>
 
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -2204,6 +2205,17 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *__restrict__
     for (int q = 0; q < NRED; q++) out[q] = sh[q][0];
 }
 
+// Scratch of tslb_reduce_noalloc: the per-block partial sums of
+// k_reduce_part, in a file-scope device array so that the entry point
+// allocates nothing.  Its grid is capped at kScratchBlocks and strides over
+// any lattice.  Every GPU the library runs on holds its own copy (57 KiB) at
+// its own address until the process ends; the host never uses the array's
+// name as an address, only the pointer hipGetSymbolAddress returns for the
+// current device (scratch_setup, called by tslb_create).
+constexpr int kScratchBlocks = 1024;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchBlocks * NRED];
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -2212,6 +2224,23 @@ thread_local std::string g_err;
 int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
+}
+
+// device address of `scratch` on each device ordinal, looked up once with
+// that device current
+double *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(TSLB_E_ARG, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return TSLB_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(TSLB_E_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch[dev] = (double *)ptr;
+  return TSLB_OK;
 }
 
 #define HIPCK(expr)                                                                 \
@@ -2577,6 +2606,10 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   HIPCK(hipGetDeviceCount(&ndev));
   if (p->device < 0 || p->device >= ndev) return fail(TSLB_E_ARG, "device ordinal out of range");
   HIPCK(hipSetDevice(p->device));
+  {
+    const int rc0 = scratch_setup(p->device);
+    if (rc0) return rc0;
+  }
 
   tslb_ctx *c = new tslb_ctx();
   c->prm = *p;
@@ -3052,6 +3085,27 @@ int tslb_reduce(tslb_ctx *c, double out[7]) {
   HIPCK(hipGetLastError());
   HIPCK(hipMemcpyAsync(out, c->red, NRED * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
+  return TSLB_OK;
+}
+
+int tslb_reduce_noalloc(tslb_ctx *c, double *out_dev) {
+  int rc = check_ctx_step(c);
+  if (rc) return rc;
+  if (!out_dev) return fail(TSLB_E_ARG, "null output pointer");
+  double *part = g_scratch[c->prm.device];
+  if (!part) return fail(TSLB_E_STATE, "no scratch address for this device");
+  const long long want = ((long long)c->n + 255) / 256;
+  const int nblk = (int)(want < kScratchBlocks ? want : kScratchBlocks);
+  if (nblk < 1 || nblk > kScratchBlocks) return fail(TSLB_E_ARG, "reduction grid beyond the scratch capacity");
+  with_storage(c, [&](auto tag) {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_reduce_part<S>, dim3(nblk), dim3(256), 0, c->stream,
+                       cfields_of<S>(c->buf[c->cur], c->fs), c->links, c->has_mask, c->kp, part);
+    return 0;
+  });
+  HIPCK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, (const double *)part, nblk, out_dev);
+  HIPCK(hipGetLastError());
   return TSLB_OK;
 }
 

@@ -72,6 +72,7 @@ SIGNATURES = {
     "tslb_upload_plane": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _FP]),
     "tslb_seed_noise": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_int]),
     "tslb_reduce": (ctypes.c_int, [_P, _DP]),
+    "tslb_reduce_noalloc": (ctypes.c_int, [_P, _P]),
     "tslb_sync": (ctypes.c_int, [_P]),
     "tslb_timing_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "tslb_timing_read": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_int64)]),

@@ -200,6 +200,18 @@ int tslb_seed_noise(tslb_ctx *ctx, int field, float amp, int l);
  * (central differences).  Synchronous. */
 int tslb_reduce(tslb_ctx *ctx, double out[7]);
 
+/* tslb_reduce without allocation, host copy or synchronisation: the same
+ * seven FP64 sums of the current state, written to out_dev (7 doubles of
+ * DEVICE memory) by two kernels queued on the context's stream; read them
+ * after tslb_sync (or an event on that stream).  No counterpart in the
+ * reference: for callers that cannot allocate or block, such as code under
+ * HIP graph capture and tight serving loops.  The per-block partials live in
+ * a fixed device array of the library (1024 blocks x 7 doubles per GPU, held
+ * until the process ends); the kernels stride over any lattice, so no
+ * lattice exceeds it.  Calls that overlap on one GPU share that array and
+ * race: serialise them (one stream, or events). */
+int tslb_reduce_noalloc(tslb_ctx *ctx, double *out_dev);
+
 /* Wait for all queued work of the context. */
 int tslb_sync(tslb_ctx *ctx);
 
