@@ -803,8 +803,11 @@ __device__ __forceinline__ void eval_ring(const Src<NC, CLO> &s, const Params &p
 struct Acc {
   float s0, sx, sy, sz, sxx, syy, szz, sxy, sxz, syz, phi;
 };
+// The sums start at -0: -0 + x is x for every x, so the compiler drops the
+// first add of each sum, where +0 + x (which turns -0 into +0) costs one
+// instruction per field and plane.
 __device__ __forceinline__ void acc_zero(Acc &a) {
-  a.s0 = a.sx = a.sy = a.sz = a.sxx = a.syy = a.szz = a.sxy = a.sxz = a.syz = a.phi = 0.f;
+  a.s0 = a.sx = a.sy = a.sz = a.sxx = a.syy = a.szz = a.sxy = a.sxz = a.syz = a.phi = -0.f;
 }
 
 // accumulate the 9 populations of one c_z layer, p[cy+1][cx+1]
@@ -896,7 +899,7 @@ __device__ __forceinline__ void colour_rewrite(float (*lds)[NSLOT], int slot, co
 template <int CZ, bool MASKED>
 __device__ __forceinline__ float sum_layer(const float (*lds)[NSLOT], int tx, int ty, float own,
                                            uint32_t bm) {
-  float s = 0.f;
+  float s = -0.f;
 #pragma unroll
   for (int cy = -1; cy <= 1; cy++)
 #pragma unroll
@@ -926,18 +929,25 @@ __device__ __forceinline__ float bb_sum_layer(const float (*lds)[NSLOT], int tx,
   return s;
 }
 
-// Runs of interface tile-planes are cut at every kRunMax-th plane of a
-// z-chunk, so that no block of the curvature pass marches more than kRunMax
-// planes: the long runs of a chunk otherwise set the length of the launch.
-constexpr int kRunMax = 8;
-constexpr uint64_t kRunCut = 0x0101010101010101ull;  // every 8th plane of a chunk
-static_assert(kRunMax == 8, "kRunCut marks every 8th plane");
+// Runs of interface tile-planes are cut at every 8th plane of a z-chunk, so
+// that no block of the curvature pass marches more than 8 planes: the long
+// runs of a chunk otherwise set the length of the launch.  A chunk with
+// interface in at least half its planes (dense emulsions) is cut at every
+// 32nd plane instead: there the short runs' start-up (five phi planes and two
+// normal planes before the first kappa) would dominate, and the many long
+// runs still balance over the blocks.
+constexpr uint64_t kRunCut8 = 0x0101010101010101ull;   // every 8th plane of a chunk
+constexpr uint64_t kRunCut32 = 0x0000000100000001ull;  // every 32nd
+__device__ __forceinline__ int run_cut(uint64_t all) { return __popcll(all) >= 32 ? 32 : 8; }
 // starts of the runs in a 64-plane hit mask, and the length of the run at k
-__device__ __forceinline__ uint64_t run_starts(uint64_t all) { return (all & ~(all << 1)) | (all & kRunCut); }
+__device__ __forceinline__ uint64_t run_starts(uint64_t all) {
+  return (all & ~(all << 1)) | (all & (run_cut(all) == 32 ? kRunCut32 : kRunCut8));
+}
 __device__ __forceinline__ int run_len(uint64_t all, int k) {
   const uint64_t above = ~(all >> k);  // first clear bit at or above k ends the run
   const int len = above ? __ffsll((long long)above) - 1 : 64 - k;
-  return min(len, kRunMax - (k & (kRunMax - 1)));
+  const int cut = run_cut(all);
+  return min(len, cut - (k & (cut - 1)));
 }
 
 template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false, bool R9 = true>
