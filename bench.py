@@ -107,7 +107,7 @@ def main():
                     help="emulsion (2C, the named workload), channel (1C, n x n/2 x n/2), tgv (1C, n^3), "
                          "colour (colour-gradient sphere, 256^3), "
                          "closure-les (learned closure on n/4 against the filtered n^3 run)")
-    ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16", "bf16"),
+    ap.add_argument("--storage", default="fp32", choices=("fp32", "fp16", "bf16", "fx16"),
                     help="moment storage precision (compute is FP32)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)

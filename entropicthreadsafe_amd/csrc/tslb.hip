@@ -40,6 +40,7 @@
 #include <sys/sysinfo.h>
 
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <utility>
 #include <vector>
@@ -131,19 +132,28 @@ struct Params {
   float tau;                // relaxation time (learned closure: omega = 1/(tau + 3 nu_t))
   float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
   float isc[TSLB_NFIELDS];  // 1 / sc
+  uint32_t step;            // fused launches so far (FX16: the dither of the stored rho - 1 and phi)
+  unsigned int *sat;        // FX16: count of stored values clipped at their range (device)
 };
 
 
-// Moment storage: FP32, FP16 or BF16 (round-to-nearest-even).  Reduced
+// Moment storage: FP32, FP16, BF16 (round-to-nearest-even) or FX16.  Reduced
 // storage keeps every field pre-multiplied by a per-field scale (Params::sc)
 // so that the shifted density and Pi^neq sit in FP16's normal range; compute
-// is always FP32.
+// is always FP32.  FX16 stores each field as a 16-bit signed fixed-point
+// number v * 32767 / R_q (R_q the field's half range, sc = 32767 / R_q):
+// uniform absolute resolution R_q / 32767 instead of 11 (FP16) or 8 (BF16)
+// significant bits, rounded to nearest even and saturated at +-R_q.
 struct bf16_t {
   uint16_t b;
+};
+struct fx16_t {
+  int16_t v;
 };
 __device__ __forceinline__ float to_f(float x) { return x; }
 __device__ __forceinline__ float to_f(_Float16 x) { return (float)x; }
 __device__ __forceinline__ float to_f(bf16_t x) { return __uint_as_float((uint32_t)x.b << 16); }
+__device__ __forceinline__ float to_f(fx16_t x) { return (float)x.v; }
 template <typename S>
 __device__ __forceinline__ S from_f(float v);
 template <>
@@ -155,6 +165,23 @@ __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
   uint32_t u = __float_as_uint(v);
   u += 0x7FFFu + ((u >> 16) & 1u);
   return bf16_t{(uint16_t)(u >> 16)};
+}
+template <>
+__device__ __forceinline__ fx16_t from_f<fx16_t>(float v) {
+  return fx16_t{(int16_t)fminf(fmaxf(rintf(v), -32767.f), 32767.f)};
+}
+// FX16 with unbiased (stochastic) rounding: floor(v + r), r uniform in
+// [0, 1) from a hash of the node, the step and the field.  Round-to-nearest
+// is biased wherever a field is nearly uniform (every node of a resting
+// phase rounds the same way), which drifts the conserved sums; the fused
+// pass stores rho - 1 and phi this way, so they drift as a random walk.
+__device__ __forceinline__ fx16_t fx16_dither(float v, uint32_t node, uint32_t step, uint32_t q) {
+  uint32_t h = node * 0x9E3779B1u + step * 0x632BE5ABu + q * 0x85EBCA6Bu;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  const float r = (float)(h >> 8) * (1.f / 16777216.f);
+  return fx16_t{(int16_t)fminf(fmaxf(floorf(v + r), -32767.f), 32767.f)};
 }
 
 template <typename S>
@@ -174,8 +201,14 @@ __device__ __forceinline__ float ldf(const S *__restrict__ a, int id, const Para
 }
 template <typename S>
 __device__ __forceinline__ void stf(S *__restrict__ a, int id, float v, const Params &p, int q) {
-  if constexpr (sizeof(S) == 4) a[id] = from_f<S>(v);
-  else a[id] = from_f<S>(v * p.sc[q]);
+  if constexpr (sizeof(S) == 4) {
+    a[id] = from_f<S>(v);
+  } else {
+    if constexpr (std::is_same<S, fx16_t>::value) {
+      if (!(fabsf(v * p.sc[q]) <= 32767.5f) && p.sat) atomicAdd(p.sat, 1u);
+    }
+    a[id] = from_f<S>(v * p.sc[q]);
+  }
 }
 
 __device__ __forceinline__ int wrapc(int g, int n) {
@@ -565,6 +598,10 @@ template <>
 __device__ __forceinline__ bf16_t bld<bf16_t>(rsrc_t r, uint32_t off) {
   return bf16_t{__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)};
 }
+template <>
+__device__ __forceinline__ fx16_t bld<fx16_t>(rsrc_t r, uint32_t off) {
+  return fx16_t{(int16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)};
+}
 // CP: cache-policy bits of the store; 2 (nt) marks it non-temporal: the
 // fused pass writes the next state once and reads none of it back, and its
 // stores then leave the L2 to the halo rows the neighbouring tiles re-read
@@ -579,6 +616,10 @@ __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, _Float16 v) {
 template <int CP = 0>
 __device__ __forceinline__ void bst(rsrc_t r, uint32_t off, bf16_t v) {
   __builtin_amdgcn_raw_buffer_store_b16(v.b, r, (int)off, 0, CP);
+}
+template <int CP = 0>
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t off, fx16_t v) {
+  __builtin_amdgcn_raw_buffer_store_b16((unsigned short)v.v, r, (int)off, 0, CP);
 }
 __device__ __forceinline__ uint32_t bld_u32(rsrc_t r, uint32_t off) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
@@ -929,25 +970,21 @@ __device__ __forceinline__ float bb_sum_layer(const float (*lds)[NSLOT], int tx,
   return s;
 }
 
-// Runs of interface tile-planes are cut at every 8th plane of a z-chunk, so
-// that no block of the curvature pass marches more than 8 planes: the long
-// runs of a chunk otherwise set the length of the launch.  A chunk with
-// interface in at least half its planes (dense emulsions) is cut at every
-// 32nd plane instead: there the short runs' start-up (five phi planes and two
-// normal planes before the first kappa) would dominate, and the many long
-// runs still balance over the blocks.
-constexpr uint64_t kRunCut8 = 0x0101010101010101ull;   // every 8th plane of a chunk
-constexpr uint64_t kRunCut32 = 0x0000000100000001ull;  // every 32nd
-__device__ __forceinline__ int run_cut(uint64_t all) { return __popcll(all) >= 32 ? 32 : 8; }
+// Runs of interface tile-planes are cut at every kRunMax-th plane of a
+// z-chunk, so that no block of the curvature pass marches more than kRunMax
+// planes: the long runs of a chunk otherwise set the length of the launch.
+// (Cutting the runs of chunks with interface in half their planes at every
+// 32nd plane instead, to save the runs' start-up, measured slower at 1.2 and
+// 15 % volume fraction: 0.142 -> 0.168 and 0.97 -> 1.02 ms per step.)
+constexpr int kRunMax = 8;
+constexpr uint64_t kRunCut = 0x0101010101010101ull;  // every 8th plane of a chunk
+static_assert(kRunMax == 8, "kRunCut marks every 8th plane");
 // starts of the runs in a 64-plane hit mask, and the length of the run at k
-__device__ __forceinline__ uint64_t run_starts(uint64_t all) {
-  return (all & ~(all << 1)) | (all & (run_cut(all) == 32 ? kRunCut32 : kRunCut8));
-}
+__device__ __forceinline__ uint64_t run_starts(uint64_t all) { return (all & ~(all << 1)) | (all & kRunCut); }
 __device__ __forceinline__ int run_len(uint64_t all, int k) {
   const uint64_t above = ~(all >> k);  // first clear bit at or above k ends the run
   const int len = above ? __ffsll((long long)above) - 1 : 64 - k;
-  const int cut = run_cut(all);
-  return min(len, cut - (k & (cut - 1)));
+  return min(len, kRunMax - (k & (kRunMax - 1)));
 }
 
 template <int NC, bool MASK, bool ADD, int WPE, typename S, bool CLO = false, bool R9 = true>
@@ -1305,11 +1342,27 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
                                aA.sxz - rho * u * w,
                                aA.syz - rho * v * w,
                                NC == 3 ? fmaf(2.f * aA.phi, inv, -1.f) : aA.phi};
+        if constexpr (std::is_same<S, fx16_t>::value) {
+          // count the nodes with a value beyond its field's range (clipped)
+          bool clip = false;
+#pragma unroll
+          for (int q = 0; q < NF; q++) clip |= !(fabsf(out[q] * p.sc[q]) <= 32767.5f);
+          if (clip && p.sat) atomicAdd(p.sat, 1u);
+        }
 #pragma unroll
         for (int q = 0; q < NF; q++) {
           const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
-          if constexpr (sizeof(S) == 4) bst<2>(r, offS, from_f<S>(out[q]));
-          else bst<2>(r, offS, from_f<S>(out[q] * p.sc[q]));
+          if constexpr (sizeof(S) == 4) {
+            bst<2>(r, offS, from_f<S>(out[q]));
+          } else if constexpr (std::is_same<S, fx16_t>::value) {
+            // the conserved fields with unbiased rounding
+            if (q == 0 || q == TSLB_PHI)
+              bst<2>(r, offS, fx16_dither(out[q] * p.sc[q], d_xy + (uint32_t)plane * (uint32_t)zd, p.step, q));
+            else
+              bst<2>(r, offS, from_f<S>(out[q] * p.sc[q]));
+          } else {
+            bst<2>(r, offS, from_f<S>(out[q] * p.sc[q]));
+          }
         }
         if constexpr (NC == 2) hit = out[10] * (1.f - out[10]) > 0.19f;
       }
@@ -2284,6 +2337,7 @@ struct tslb_ctx {
   int normals_fresh = 0;
   double *part = nullptr, *red = nullptr;
   float *w_in = nullptr, *phi_in = nullptr;
+  unsigned int *sat = nullptr;  // FX16: clipped-value counter (kp.sat)
   int zbc = 0, zbc_phi = 0;
   int jet = 0;     // turbulent-jet inlet/outlet planes (k_jet_bc after k_fused)
   int zwalls = 0;  // bit 0: plane 0 all solid; bit 1: plane nz-1 all solid
@@ -2305,12 +2359,25 @@ struct tslb_ctx {
 
 namespace {
 
+// Default half ranges of the FX16 fields: rho - 1 within +-1/4 (the start-up
+// pressure waves of a sharp interface reach ~0.1), velocities within +-0.375
+// (beyond the lattice's stable range), Pi^neq within +-1/8, phi (or the
+// colour psi) within +-1.125.  Resolution = range / 32767.  Clipped values
+// are counted (tslb_storage_clipped).
+float fx16_range(int q) {
+  if (q == TSLB_RHO) return 0.25f;
+  if (q >= TSLB_U && q <= TSLB_W) return 0.375f;
+  if (q >= TSLB_PXX && q <= TSLB_PYZ) return 0.125f;
+  return 1.125f;
+}
+
 // call fn with a value of the storage type of the context
 template <typename Fn>
 auto with_storage(const tslb_ctx *c, Fn &&fn) {
   switch (c->prm.storage) {
     case TSLB_STORAGE_FP16: return fn(_Float16{});
     case TSLB_STORAGE_BF16: return fn(bf16_t{});
+    case TSLB_STORAGE_FX16: return fn(fx16_t{});
     default: return fn(float{});
   }
 }
@@ -2527,6 +2594,7 @@ int launch_fused(tslb_ctx *c) {
   }
   // one component: phi stays in buf[phi_buf] (check_ctx moves it when needed)
   c->cur = 1 - c->cur;
+  c->kp.step += 1;
   if (build) {
     // the next curvature pass writes the flags the fused pass just cleared
     std::swap(c->flags, c->flags_alt);
@@ -2608,8 +2676,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     return fail(TSLB_E_ARG, "tau_b must be > 0.5 (or <= 0 for tau)");
   if (!(p->tau > 0.5f)) return fail(TSLB_E_ARG, "tau must be > 0.5");
   if (p->storage != TSLB_STORAGE_FP32 && p->storage != TSLB_STORAGE_FP16 &&
-      p->storage != TSLB_STORAGE_BF16)
-    return fail(TSLB_E_ARG, "storage must be TSLB_STORAGE_FP32, _FP16 or _BF16");
+      p->storage != TSLB_STORAGE_BF16 && p->storage != TSLB_STORAGE_FX16)
+    return fail(TSLB_E_ARG, "storage must be TSLB_STORAGE_FP32, _FP16, _BF16 or _FX16");
   if (p->force_mode != TSLB_FORCE_ADDITIVE && p->force_mode != TSLB_FORCE_SHIFT)
     return fail(TSLB_E_ARG, "force_mode must be TSLB_FORCE_ADDITIVE or TSLB_FORCE_SHIFT");
   int ndev = 0;
@@ -2668,7 +2736,9 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   }
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
   // storage scales: FP16 keeps rho-1 (x64) and Pi^neq (x4096) in its normal
-  // range; BF16 has FP32's exponent range and FP32 needs none
+  // range; BF16 has FP32's exponent range and FP32 needs none; FX16 maps each
+  // field's half range onto 32767 (default ranges in fx16_range, changed by
+  // tslb_set_storage_range)
   c->es = p->storage == TSLB_STORAGE_FP32 ? 4 : 2;
   for (int q = 0; q < TSLB_NFIELDS; q++) {
     float sc = 1.f;
@@ -2676,6 +2746,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
       if (q == TSLB_RHO) sc = 64.f;
       if (q >= TSLB_PXX && q <= TSLB_PYZ) sc = 4096.f;
     }
+    if (p->storage == TSLB_STORAGE_FX16) sc = 32767.f / fx16_range(q);
     k.sc[q] = sc;
     k.isc[q] = 1.f / sc;
   }
@@ -2696,6 +2767,7 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
        (rc = dev_alloc(c, (void **)&c->ilist, ((size_t)k.tiles_x * k.tiles_y * p->nz + 1) * sizeof(int)))) ||
       (p->ncomp == 2 &&
        (rc = dev_alloc(c, (void **)&c->flags_alt, (size_t)k.tiles_x * k.tiles_y * p->nz))) ||
+      (p->storage == TSLB_STORAGE_FX16 && (rc = dev_alloc(c, (void **)&c->sat, 256))) ||
       (rc = dev_alloc(c, (void **)&c->part, (size_t)kRedBlocks * NRED * sizeof(double))) ||
       (rc = dev_alloc(c, (void **)&c->red, NRED * sizeof(double)))) {
     std::string msg = g_err;
@@ -2704,6 +2776,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   }
   // zero state: rho-1 = 0, all moments 0
   for (int b = 0; b < 2; b++) zero_fields(c, c->buf[b], 0, TSLB_NFIELDS);
+  if (c->sat) hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, c->stream, (float *)c->sat, 0.f, 1);
+  k.sat = c->sat;
   if (c->nrm) {
     const int tot = (int)(4 * c->fs);
     hipLaunchKernelGGL(k_fill, dim3((tot + 255) / 256), dim3(256), 0, c->stream, c->nrm, 0.f, tot);
@@ -2738,6 +2812,7 @@ int tslb_destroy(tslb_ctx *c) {
   (void)hipFree(c->part);
   (void)hipFree(c->red);
   (void)hipFree(c->w_in);
+  (void)hipFree(c->sat);
   (void)hipFree(c->phi_in);
   (void)hipFree(c->clo);
   (void)hipFree(c->cfeat);
@@ -2857,7 +2932,8 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   CkptHeader h;
   memset(&h, 0, sizeof(h));
   memcpy(h.magic, "TSLBCKPT", 8);
-  h.version = 1;
+  // version 2: FX16 storage, whose 11 field scales follow the header
+  h.version = c->prm.storage == TSLB_STORAGE_FX16 ? 2 : 1;
   h.nx = c->prm.nx;
   h.ny = c->prm.ny;
   h.nz = c->prm.nz;
@@ -2871,6 +2947,8 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   const size_t bytes = c->n * c->es;
   std::vector<char> host(bytes);
   bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
+  if (ok && h.version == 2) ok = fwrite(c->kp.sc, sizeof(float), TSLB_NFIELDS, fp) == (size_t)TSLB_NFIELDS;
+  if (ok && h.version == 2) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
   for (int q = 0; ok && q < TSLB_NFIELDS; q++) {
     hipError_t e = hipMemcpyAsync(host.data(), (char *)c->buf[c->cur] + (size_t)q * c->fs * c->es, bytes,
                                   hipMemcpyDeviceToHost, c->stream);
@@ -2893,7 +2971,8 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
   FILE *fp = fopen(path, "rb");
   if (!fp) return fail(TSLB_E_ARG, std::string("cannot open ") + path);
   CkptHeader h;
-  if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0 || h.version != 1) {
+  if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0 ||
+      h.version != (h.storage == TSLB_STORAGE_FX16 ? 2 : 1)) {
     fclose(fp);
     return fail(TSLB_E_ARG, std::string(path) + " is not a TSLB checkpoint");
   }
@@ -2901,6 +2980,26 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
       h.storage != c->prm.storage || h.elem_bytes != (int32_t)c->es || h.nfields != TSLB_NFIELDS) {
     fclose(fp);
     return fail(TSLB_E_ARG, "checkpoint lattice, components or storage differ from the context");
+  }
+  float sc[TSLB_NFIELDS];
+  uint32_t kstep = 0;
+  if (h.version == 2 && (fread(sc, sizeof(float), TSLB_NFIELDS, fp) != (size_t)TSLB_NFIELDS ||
+                         fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
+    fclose(fp);
+    return fail(TSLB_E_ARG, std::string("truncated checkpoint ") + path);
+  }
+  if (h.version == 2) {
+    for (int q = 0; q < TSLB_NFIELDS; q++) {
+      if (!(sc[q] > 0.f) || !(sc[q] < 1e30f)) {
+        fclose(fp);
+        return fail(TSLB_E_ARG, std::string(path) + " holds an invalid FX16 scale");
+      }
+    }
+    for (int q = 0; q < TSLB_NFIELDS; q++) {
+      c->kp.sc[q] = sc[q];
+      c->kp.isc[q] = 1.f / sc[q];
+    }
+    c->kp.step = kstep;
   }
   const size_t bytes = c->n * c->es;
   std::vector<char> host(bytes);
@@ -3060,6 +3159,42 @@ int tslb_set_zplane_bc(tslb_ctx *c, const float *w_in, const float *phi_in) {
   if ((rc = launch_zbc(c))) return rc;
   HIPCK(hipStreamSynchronize(c->stream));
   c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int tslb_set_storage_range(tslb_ctx *c, int field, float half_range) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->prm.storage != TSLB_STORAGE_FX16) return fail(TSLB_E_STATE, "storage ranges apply to TSLB_STORAGE_FX16");
+  if (field < 0 || field >= TSLB_NFIELDS) return fail(TSLB_E_ARG, "bad field");
+  if (!(half_range > 0.f) || !(half_range < 1e30f)) return fail(TSLB_E_ARG, "half_range must be positive and finite");
+  // re-encode the stored field: decode with the old scale into the second
+  // buffer (scratch between steps), encode with the new one
+  float *tmp = (float *)c->buf[1 - c->cur];
+  fx16_t *f = field_ptr<fx16_t>(c->buf[c->cur], c->fs, field);
+  const dim3 g((unsigned)((c->n + 255) / 256));
+  hipLaunchKernelGGL(k_from_storage<fx16_t>, g, dim3(256), 0, c->stream, (const fx16_t *)f, tmp, (int)c->n, field,
+                     c->kp);
+  HIPCK(hipGetLastError());
+  c->kp.sc[field] = 32767.f / half_range;
+  c->kp.isc[field] = 1.f / c->kp.sc[field];
+  hipLaunchKernelGGL(k_to_storage<fx16_t>, g, dim3(256), 0, c->stream, (const float *)tmp, f, (int)c->n, field, c->kp);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->normals_fresh = 0;
+  return TSLB_OK;
+}
+
+int tslb_storage_clipped(tslb_ctx *c, int64_t *count) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!count) return fail(TSLB_E_ARG, "null pointer");
+  *count = 0;
+  if (!c->sat) return TSLB_OK;
+  unsigned int v = 0;
+  HIPCK(hipMemcpyAsync(&v, c->sat, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  *count = (int64_t)v;
   return TSLB_OK;
 }
 

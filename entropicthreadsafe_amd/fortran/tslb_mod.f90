@@ -12,7 +12,8 @@ module tslb_mod
   integer(c_int), parameter :: TSLB_PXY = 7, TSLB_PXZ = 8, TSLB_PYZ = 9, TSLB_PHI = 10
   integer(c_int), parameter :: TSLB_FORCE_ADDITIVE = 0, TSLB_FORCE_SHIFT = 1
   integer(c_int), parameter :: TSLB_COLOUR = 3
-  integer(c_int), parameter :: TSLB_STORAGE_FP32 = 0, TSLB_STORAGE_BF16 = 1, TSLB_STORAGE_FP16 = 2
+  integer(c_int), parameter :: TSLB_STORAGE_FP32 = 0, TSLB_STORAGE_BF16 = 1, TSLB_STORAGE_FP16 = 2, &
+    TSLB_STORAGE_FX16 = 3
 
   type, bind(C) :: tslb_params
     integer(c_int32_t) :: nx, ny, nz
@@ -71,6 +72,12 @@ module tslb_mod
       type(c_ptr), value :: ctx
       integer(c_int), value :: field, l
       real(c_float), value :: amp
+    end function
+    integer(c_int) function tslb_set_storage_range(ctx, field, half_range) bind(C, name='tslb_set_storage_range')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      real(c_float), value :: half_range
     end function
     integer(c_int) function tslb_step(ctx, nsteps) bind(C, name='tslb_step')
       import :: c_int, c_ptr

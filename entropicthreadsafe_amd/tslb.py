@@ -22,7 +22,7 @@ NFIELDS = 11
 
 FORCE_ADDITIVE = 0
 FORCE_SHIFT = 1
-STORAGE = {"fp32": 0, "bf16": 1, "fp16": 2}
+STORAGE = {"fp32": 0, "bf16": 1, "fp16": 2, "fx16": 3}
 
 
 class TslbError(RuntimeError):
@@ -71,6 +71,8 @@ SIGNATURES = {
     "tslb_checkpoint_read": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "tslb_upload_plane": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _FP]),
     "tslb_seed_noise": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_int]),
+    "tslb_set_storage_range": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float]),
+    "tslb_storage_clipped": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "tslb_reduce": (ctypes.c_int, [_P, _DP]),
     "tslb_reduce_noalloc": (ctypes.c_int, [_P, _P]),
     "tslb_sync": (ctypes.c_int, [_P]),
@@ -288,6 +290,17 @@ class Lattice:
         if a.size != self.shape[1] * self.shape[2]:
             raise ValueError("plane has the wrong size")
         _check(self.lib.tslb_upload_plane(self._h, q, int(k), _fptr(a)), "tslb_upload_plane")
+
+    def set_storage_range(self, name, half_range):
+        """FX16 storage: half range of one field (values beyond saturate)."""
+        q = FIELDS.index(name) if isinstance(name, str) else int(name)
+        _check(self.lib.tslb_set_storage_range(self._h, q, float(half_range)), "tslb_set_storage_range")
+
+    def storage_clipped(self):
+        """FX16 storage: values clipped at their range since creation."""
+        n = ctypes.c_int64()
+        _check(self.lib.tslb_storage_clipped(self._h, ctypes.byref(n)), "tslb_storage_clipped")
+        return n.value
 
     def seed_noise(self, name, amp, l):
         """field += amp*(2 r - 1), r = rand_noseeded(i, j, k, l), on the device."""

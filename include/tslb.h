@@ -46,6 +46,12 @@ extern "C" {
 #define TSLB_STORAGE_FP32 0
 #define TSLB_STORAGE_BF16 1
 #define TSLB_STORAGE_FP16 2
+/* 16-bit signed fixed point per field: value * 32767 / R (R the field's half
+ * range; defaults rho-1 1/4, u 0.375, Pi^neq 1/8, phi 1.125; see
+ * tslb_set_storage_range), clipped at +-R (tslb_storage_clipped counts it).  The step rounds rho-1 and phi
+ * stochastically (floor(x + r), r from a hash of node, step and field), the
+ * other fields and all uploads to nearest. */
+#define TSLB_STORAGE_FX16 3
 
 /* Moment field indices (the order of tslb_upload_moments' packed buffer). */
 #define TSLB_RHO 0
@@ -118,8 +124,11 @@ int tslb_download_field(tslb_ctx *ctx, int field, float *data);
 /* Moment-space checkpoint, in place of the population dump and reload
  * dump_distros_1c_3d / read_distros_1c_3d (print_module.f90:700-723).  The
  * file holds a 64-byte header (magic "TSLBCKPT", version 1, nx, ny, nz,
- * ncomp, storage, bytes per value, field count, the caller's step) and the
- * 11 stored fields verbatim (storage type, field 0 as rho-1), so a run that
+ * ncomp, storage, bytes per value, field count, the caller's step; version 2
+ * for FX16 storage, whose 11 float field scales and the library's step count
+ * (uint32, the rounding dither) follow the header and are restored on
+ * reading) and the 11 stored fields verbatim (storage type,
+ * field 0 as rho-1), so a run that
  * reads it continues bit for bit as one that never stopped.  The isfluid map
  * and plane BCs are not part of it: the caller sets them again.  Reading
  * checks lattice, component count and storage against the context. */
@@ -188,6 +197,17 @@ int tslb_set_jet_bc(tslb_ctx *ctx, int enable);
  * Replaces `!$acc update device(w(...,1))` of the inflow refresh
  * (boundary_cds_3D_module.f90:866-877). */
 int tslb_upload_plane(tslb_ctx *ctx, int field, int k, const float *data);
+
+/* FX16 storage: the half range R of one field (resolution R / 32767; values
+ * beyond +-R saturate).  The stored field is re-encoded.  No counterpart in
+ * the reference (which stores FP32, vars_single_3D_module.f90:4): the
+ * reduced-precision storage of the moment fields. */
+int tslb_set_storage_range(tslb_ctx *ctx, int field, float half_range);
+
+/* FX16 storage: how many stored values were clipped at their range since the
+ * context was created (the fused pass counts nodes, uploads count values);
+ * 0 for other storage.  Synchronous.  No counterpart in the reference. */
+int tslb_storage_clipped(tslb_ctx *ctx, int64_t *count);
 
 /* Seeded noise: field += amp * (2 r - 1) with r = tslb_rand_noseeded(i, j, k, l)
  * at 1-based node coordinates, computed on the device.  Replaces the seeded

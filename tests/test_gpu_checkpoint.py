@@ -9,11 +9,14 @@ from entropicthreadsafe_amd.tslb import Lattice, TslbError
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16", "fx16"])
 def test_restart_is_bitwise(tmp_path, storage):
     case = configs.droplet(n=24, radius=6.0, sigma=0.02, beta=0.7, noise=1e-3)
     ck = str(tmp_path / "state.ckpt")
     with Lattice(24, 24, 24, storage=storage, **case.lattice) as lat:
+        if storage == "fx16":
+            # a non-default range travels with the checkpoint
+            lat.set_storage_range("u", 0.05)
         case.apply(lat)
         lat.step(7)
         lat.checkpoint_write(ck, step=7)

@@ -35,6 +35,8 @@ TOL_SMALL = {
                  rms=dict(rho=1.4e-5, u=9e-3, p=1.3e-2, phi=8e-4)),
     "bf16": dict(max=dict(rho=1.6e-3, u=0.31, p=0.43, phi=0.105),
                  rms=dict(rho=4e-5, u=2.6e-2, p=3.5e-2, phi=1e-2)),
+    "fx16": dict(max=dict(rho=1.3e-3, u=0.23, p=0.29, phi=7e-3),
+                 rms=dict(rho=1.4e-5, u=9e-3, p=1.3e-2, phi=8e-4)),
 }
 GROUPS = dict(rho=("rho",), u=("u", "v", "w"), p=("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"), phi=("phi",))
 
@@ -81,7 +83,7 @@ def test_small_emulsion_fp32_matches_fp64_oracle():
     assert abs(r1[1] - r0[1]) <= 1e-7 * r0[1]
 
 
-@pytest.mark.parametrize("storage", ["fp16", "bf16"])
+@pytest.mark.parametrize("storage", ["fp16", "bf16", "fx16"])
 def test_small_emulsion_reduced_storage(storage):
     case = small_emulsion()
     got, ref, r0, r1 = _run(case, 15, storage)
@@ -103,6 +105,7 @@ def test_small_emulsion_reduced_storage(storage):
 @pytest.mark.parametrize("storage,mass_tol,phase_tol", [
     ("fp32", 1e-9, 1e-8),
     ("bf16", 1e-8, 2.3e-2),
+    ("fx16", 1e-8, 1e-3),
 ])
 def test_emulsion_512_long_run(storage, mass_tol, phase_tol):
     case = configs.emulsion()  # 512^3, 200 droplets, radii U[8, 16]
