@@ -171,16 +171,17 @@ __device__ __forceinline__ fx16_t from_f<fx16_t>(float v) {
   return fx16_t{(int16_t)fminf(fmaxf(rintf(v), -32767.f), 32767.f)};
 }
 // FX16 with unbiased (stochastic) rounding: floor(v + r), r uniform in
-// [0, 1) from a hash of the node, the step and the field.  Round-to-nearest
-// is biased wherever a field is nearly uniform (every node of a resting
-// phase rounds the same way), which drifts the conserved sums; the fused
-// pass stores rho - 1 and phi this way, so they drift as a random walk.
-__device__ __forceinline__ fx16_t fx16_dither(float v, uint32_t node, uint32_t step, uint32_t q) {
-  uint32_t h = node * 0x9E3779B1u + step * 0x632BE5ABu + q * 0x85EBCA6Bu;
-  h ^= h >> 15;
-  h *= 0x2C1B3C6Du;
-  h ^= h >> 12;
-  const float r = (float)(h >> 8) * (1.f / 16777216.f);
+// [0, 1).  Round-to-nearest is biased wherever a field is nearly uniform
+// (every node of a resting phase rounds the same way), which drifts the
+// conserved sums; the fused pass stores rho - 1 and phi this way, so they
+// drift as a random walk.  One multiplicative hash of the node and a per-step
+// seed gives both offsets: rho - 1 takes its high 24 bits, phi its low 24.
+__device__ __forceinline__ uint32_t fx16_hash(uint32_t node, uint32_t seed) {
+  const uint32_t h = (node ^ seed) * 0x9E3779B1u;
+  return h ^ (h >> 16);
+}
+__device__ __forceinline__ fx16_t fx16_dither(float v, uint32_t bits24) {
+  const float r = (float)bits24 * (1.f / 16777216.f);
   return fx16_t{(int16_t)fminf(fmaxf(floorf(v + r), -32767.f), 32767.f)};
 }
 
@@ -1349,6 +1350,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           for (int q = 0; q < NF; q++) clip |= !(fabsf(out[q] * p.sc[q]) <= 32767.5f);
           if (clip && p.sat) atomicAdd(p.sat, 1u);
         }
+        uint32_t hd = 0;  // FX16: the dither of this node and step
+        if constexpr (std::is_same<S, fx16_t>::value)
+          hd = fx16_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step * 0x632BE5ABu);
 #pragma unroll
         for (int q = 0; q < NF; q++) {
           const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
@@ -1356,8 +1360,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
             bst<2>(r, offS, from_f<S>(out[q]));
           } else if constexpr (std::is_same<S, fx16_t>::value) {
             // the conserved fields with unbiased rounding
-            if (q == 0 || q == TSLB_PHI)
-              bst<2>(r, offS, fx16_dither(out[q] * p.sc[q], d_xy + (uint32_t)plane * (uint32_t)zd, p.step, q));
+            if (q == 0)
+              bst<2>(r, offS, fx16_dither(out[q] * p.sc[q], hd >> 8));
+            else if (q == TSLB_PHI)
+              bst<2>(r, offS, fx16_dither(out[q] * p.sc[q], hd & 0xFFFFFFu));
             else
               bst<2>(r, offS, from_f<S>(out[q] * p.sc[q]));
           } else {
@@ -2362,13 +2368,14 @@ namespace {
 // Default half ranges of the FX16 fields: rho - 1 within +-1/4 (the start-up
 // pressure waves of a sharp interface reach ~0.1), velocities within +-0.375
 // (beyond the lattice's stable range), Pi^neq within +-1/8, phi (or the
-// colour psi) within +-1.125.  Resolution = range / 32767.  Clipped values
+// colour psi) within +-1.5 (a sharp droplet's start-up overshoots phi to
+// ~1.15).  Resolution = range / 32767.  Clipped values
 // are counted (tslb_storage_clipped).
 float fx16_range(int q) {
   if (q == TSLB_RHO) return 0.25f;
   if (q >= TSLB_U && q <= TSLB_W) return 0.375f;
   if (q >= TSLB_PXX && q <= TSLB_PYZ) return 0.125f;
-  return 1.125f;
+  return 1.5f;
 }
 
 // call fn with a value of the storage type of the context

@@ -3,11 +3,11 @@ seeded velocity noise; configs.emulsion) on the GPU.
 
   * A reduced emulsion (64^3, 8 droplets with radii U[4, 8]) against the FP64
     oracle, pointwise over 15 steps: FP32 storage within 1e-5 per field (the
-    two-component horizon of tests/test_gpu_parity.py), FP16 and BF16
+    two-component horizon of tests/test_gpu_parity.py), FP16, BF16 and FX16
     storage within the bounds below (about 1.5x the measured errors).
   * The workload at its size (512^3, 200 droplets with radii U[8, 16]) over
-    200 steps in FP32 and BF16 storage: finite state, mass drift (FP64 sums)
-    and phase-mass drift within stated bounds.
+    200 steps in FP32, BF16 and FX16 storage: finite state, mass drift (FP64
+    sums) and phase-mass drift within stated bounds.
 
 Reference step: recursive_2c_TSLB3D.f90:344-840; linear forcing enters the
 two-component equilibrium as a velocity shift (TSLB_2C-mpi/lb_kernels_module.
@@ -35,8 +35,8 @@ TOL_SMALL = {
                  rms=dict(rho=1.4e-5, u=9e-3, p=1.3e-2, phi=8e-4)),
     "bf16": dict(max=dict(rho=1.6e-3, u=0.31, p=0.43, phi=0.105),
                  rms=dict(rho=4e-5, u=2.6e-2, p=3.5e-2, phi=1e-2)),
-    "fx16": dict(max=dict(rho=1.3e-3, u=0.23, p=0.29, phi=7e-3),
-                 rms=dict(rho=1.4e-5, u=9e-3, p=1.3e-2, phi=8e-4)),
+    "fx16": dict(max=dict(rho=1.1e-4, u=6.3e-2, p=4.3e-2, phi=2e-3),
+                 rms=dict(rho=1.1e-5, u=5.7e-3, p=7.4e-3, phi=2.2e-4)),
 }
 GROUPS = dict(rho=("rho",), u=("u", "v", "w"), p=("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"), phi=("phi",))
 

@@ -3,7 +3,7 @@ the FP64 oracle.
 
 The state is stored with a shifted density (rho - 1) and, for FP16, scaled
 rho-1 (x64) and Pi^neq (x4096); FX16 stores 16-bit fixed point over a half
-range per field (rho-1 1/4, u 0.375, Pi^neq 1/8, phi 1.125); every kernel
+range per field (rho-1 1/4, u 0.375, Pi^neq 1/8, phi 1.5); every kernel
 computes in FP32.  Errors are the
 per-field max relative errors of tests/parity.py (RMS errors are printed too).
 
@@ -11,7 +11,7 @@ Each step rounds every stored field once, and Pi^neq at tau ~ 0.51 decays by
 only |1 - omega| ~ 0.93-0.96 per step, so rounding accumulates over ~15-25
 steps.  Measured on the MI355X (max over TGV 32^3 x 100 steps and channel x
 300 steps; rho, u, Pi): FP16 1.2e-4, 1.6e-3, 1.4e-2; BF16 (8 significant
-bits) 5.2e-4, 2.9e-2, 0.12; FX16 3.5e-5, 7.6e-4, 5.5e-3.  The bounds are 1.5x
+bits) 5.2e-4, 2.9e-2, 0.12; FX16 3.4e-5, 7.6e-4, 5.7e-3.  The bounds are 1.5x
 those, so a broken codec or scale fails them.
 """
 import numpy as np
@@ -26,11 +26,11 @@ pytestmark = pytest.mark.gpu
 
 # (rho, u, Pi) bounds per storage: 1.5x the larger of the errors measured on
 # the MI355X in the Taylor-Green and channel cases below
-TOL = {"fp16": (1.8e-4, 2.4e-3, 2.1e-2), "bf16": (7.9e-4, 4.3e-2, 0.18), "fx16": (1e-4, 1.5e-3, 2e-2)}
+TOL = {"fp16": (1.8e-4, 2.4e-3, 2.1e-2), "bf16": (7.9e-4, 4.3e-2, 0.18), "fx16": (5.1e-5, 1.2e-3, 8.5e-3)}
 # the same at the C3 size, where Pi^neq carries the noise of the stored
-# velocity's rounding (test_c3_channel_storage_error); FX16 is held to the
-# 1e-3 target for rho and u
-TOL_C3 = {"fp16": (2.2e-4, 3.2e-3, 0.2), "bf16": (2.0e-3, 3.6e-2, 2.0), "fx16": (1e-3, 1e-3, 0.1)}
+# velocity's rounding (test_c3_channel_storage_error); FX16 meets the 1e-3
+# target for rho and u (measured 5.6e-5 and 4.9e-4)
+TOL_C3 = {"fp16": (2.2e-4, 3.2e-3, 0.2), "bf16": (2.0e-3, 3.6e-2, 2.0), "fx16": (8.4e-5, 7.4e-4, 9.5e-2)}
 STORAGES = ["fp16", "bf16", "fx16"]
 
 
