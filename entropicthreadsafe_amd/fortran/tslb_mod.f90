@@ -79,6 +79,11 @@ module tslb_mod
       integer(c_int), value :: field
       real(c_float), value :: half_range
     end function
+    integer(c_int) function tslb_storage_clipped(ctx, count) bind(C, name='tslb_storage_clipped')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), intent(out) :: count
+    end function
     integer(c_int) function tslb_step(ctx, nsteps) bind(C, name='tslb_step')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
