@@ -1767,9 +1767,11 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
 // gate -- the arithmetic of k_normals_tiled, so the values are the same bits.
 // Every listed tile-plane gets its kappa (zeros included) and flag byte 1,
 // so one barrier per plane suffices.  Blocks stride over the list (its
-// length read from the counter), so every wave ends its loop.
+// length read from the counter), so every wave ends its loop.  The plane
+// loop is unrolled over the longest run (ring slots at constant offsets) and
+// held to 80 VGPRs, which keeps three blocks per CU (51 KiB of LDS each).
 template <typename S>
-__global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, float *__restrict__ kap,
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_curv_list(const S *__restrict__ phi, float *__restrict__ kap,
                                                     uint8_t *__restrict__ flags, const int *__restrict__ cnt,
                                                     const int *__restrict__ list, const uint32_t *__restrict__ links,
                                                     int has_mask, Params p) {
@@ -1823,26 +1825,34 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
     const int c0 = (tid / SW + 1) * PW + tid % SW + 1;
     const int c1 = ntwo ? ((tid + NTHR) / SW + 1) * PW + (tid + NTHR) % SW + 1 : c0;
     GradRun g0, g1;
+    // per normal slot, fixed over the run: in-plane validity and the byte
+    // offset of its link word
+    auto slot_xy = [&](int sl, uint32_t &lofs) -> bool {
+      int gx = 0, gy = 0;
+      const bool ok = src_coord(x0 + sl % SW - 1, p.nx, p.px, gx) && src_coord(y0 + sl / SW - 1, p.ny, p.py, gy);
+      lofs = (uint32_t)(gx + p.nx * gy) * 4u;
+      return ok;
+    };
+    uint32_t lofs0 = 0, lofs1 = 0;
+    const bool okxy0 = slot_xy(tid, lofs0);
+    const bool okxy1 = ntwo && slot_xy(tid + NTHR, lofs1);
     auto normals = [&](int zz) {
       int gz = 0;
       const bool zin = src_coord(zz, p.nz, p.pz, gz);
       const rsrc_t lk = rsrc(links + plane * (size_t)gz, pb4);
       const float *P0 = ph[rph(zz - 1)], *P2 = ph[rph(zz + 1)];
       float(*dn)[NSLOT] = nl[rnl(zz)];
-      auto one = [&](int sl, int c, GradRun &g) {
+      auto one = [&](int sl, int c, GradRun &g, bool okxy, uint32_t lofs) {
         float nn[3];
         grad_run_step(g, plane_grad(P2, c), P0[c], nn);
-        const int sx = sl % SW, sy = sl / SW;
-        int gx = 0, gy = 0;
-        const bool okxy = src_coord(x0 + sx - 1, p.nx, p.px, gx) && src_coord(y0 + sy - 1, p.ny, p.py, gy);
         bool ok = zin && okxy;
-        if (ok && has_mask) ok = (bld_u32(lk, (uint32_t)(gx + p.nx * gy) * 4u) & kSolid) == 0u;
+        if (ok && has_mask) ok = (bld_u32(lk, lofs) & kSolid) == 0u;
         dn[0][sl] = ok ? nn[0] : 0.f;
         dn[1][sl] = ok ? nn[1] : 0.f;
         dn[2][sl] = ok ? nn[2] : 0.f;
       };
-      one(tid, c0, g0);
-      if (ntwo) one(tid + NTHR, c1, g1);
+      one(tid, c0, g0, okxy0, lofs0);
+      if (ntwo) one(tid + NTHR, c1, g1, okxy1, lofs1);
     };
     // prologue: phi planes zs-2..zs+2 (all five loads in flight at once), n
     // of planes zs-1 and zs
@@ -1859,13 +1869,18 @@ __global__ __launch_bounds__(NTHR) void k_curv_list(const S *__restrict__ phi, f
     if (ntwo) grad_run_init(g1, plane_grad(ph[rph(zs - 2)], c1), plane_grad(ph[rph(zs - 1)], c1));
     normals(zs - 1);
     normals(zs);
-    for (int z = zs; z < zs + len; z++) {
+    // unrolled over the longest run, so that every ring slot index is a
+    // constant offset from the run's start
+#pragma unroll
+    for (int k = 0; k < kRunMax; k++) {
+      if (k >= len) break;
+      const int z = zs + k;
       normals(z + 1);
       // phi of plane z+3 into the slot of plane z-3, whose last reads (n of
       // plane z-2, the gate of plane z-3) came before the previous barrier
-      if (z + 1 < zs + len) {
+      if (k + 1 < len) {
         commit(z + 3);
-        if (z + 2 < zs + len) issue(z + 4);
+        if (k + 2 < len) issue(z + 4);
       }
       __syncthreads();
       float curv = 0.f;
@@ -2338,6 +2353,7 @@ struct tslb_ctx {
   uint8_t *flags_alt = nullptr;  // phase field: flags of the next step (cleared by the fused pass)
   int list_ok = 0;           // phase field: ilist describes the current state
   int ncu = 256;             // compute units of the device
+  int curv_grid = 768;       // k_curv_list blocks: as many as are resident at once
   uint32_t *links = nullptr;
   int has_mask = 0;
   int normals_fresh = 0;
@@ -2483,7 +2499,7 @@ int launch_normals(tslb_ctx *c, int dense = 0) {
         hipLaunchKernelGGL((k_iface_list<S>), dim3(nblk), dim3(NTHR), 0, c->stream, phi, c->flags, c->ilist,
                            c->ilist + 1, c->kp);
       }
-      hipLaunchKernelGGL((k_curv_list<S>), dim3(4 * c->ncu), dim3(NTHR), 0, c->stream, phi, c->nrm + 3 * c->fs,
+      hipLaunchKernelGGL((k_curv_list<S>), dim3(c->curv_grid), dim3(NTHR), 0, c->stream, phi, c->nrm + 3 * c->fs,
                          c->flags, c->ilist, c->ilist + 1, c->links, c->has_mask, c->kp);
     }
     return 0;
@@ -2732,6 +2748,17 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     if (hipGetDeviceProperties(&prop, p->device) == hipSuccess && prop.multiProcessorCount > 0)
       ncu = prop.multiProcessorCount;
     c->ncu = ncu;
+    // the curvature blocks stride over the run list with equal shares, so
+    // the grid is the number that is resident at once: one block more per
+    // CU would start only when another ends and double the launch's length
+    int bpc = 0;
+    with_storage(c, [&](auto tag) {
+      using S = decltype(tag);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_curv_list<S>, NTHR, 0) != hipSuccess) bpc = 0;
+      return 0;
+    });
+    if (const char *v = getenv("TSLB_CURV_BPC")) bpc = atoi(v);
+    c->curv_grid = (bpc > 0 ? bpc : 3) * ncu;
     const long long tiles = (long long)k.tiles_x * k.tiles_y;
     int zc = 64;
     while (zc > 8 && tiles * ((p->nz + zc - 1) / zc) < 2LL * ncu) zc /= 2;
