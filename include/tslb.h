@@ -46,11 +46,12 @@ extern "C" {
 #define TSLB_STORAGE_FP32 0
 #define TSLB_STORAGE_BF16 1
 #define TSLB_STORAGE_FP16 2
-/* 16-bit signed fixed point per field: value * 32767 / R (R the field's half
+/* 16-bit signed fixed point per field: round(value * 32767 / R) (R the field's half
  * range; defaults rho-1 1/4, u 0.375, Pi^neq 1/8, phi 1.5; see
- * tslb_set_storage_range), clipped at +-R (tslb_storage_clipped counts it).  The step rounds rho-1 and phi
- * stochastically (floor(x + r), r from a hash of node and step), the
- * other fields and all uploads to nearest. */
+ * tslb_set_storage_range), clipped at +-R (tslb_storage_clipped counts it).
+ * The step rounds rho-1 and phi stochastically (floor(x + r), r uniform in
+ * [0, 1) from a hash of node and step), the other fields and all uploads to
+ * nearest. */
 #define TSLB_STORAGE_FX16 3
 
 /* Moment field indices (the order of tslb_upload_moments' packed buffer). */
