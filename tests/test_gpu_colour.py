@@ -147,3 +147,70 @@ def test_colour_sphere_at_config_2_size():
     assert umax < 1e-3
     assert abs(r1[0] - r0[0]) < 1e-7 * r0[0]
     assert abs(r1[1] - r0[1]) < 1e-3 * r0[1]
+
+
+# Reduced storage in the colour model: per group (rho, u, Pi, psi) max and RMS
+# error against the FP64 oracle after 15 steps of the droplet, and the Laplace
+# law (within 10 %, as in FP32) and both masses over 500 steps.  Bounds are
+# 1.5x the MI355X measurements:
+#   max (rho, u, Pi, psi)   FP16 2.3e-5 4.1e-3 1.6e-3 6.7e-4, BF16 1.9e-4
+#   2.7e-2 6.4e-3 6.8e-3, FX16 1.8e-5 8.5e-3 1.3e-3 8.1e-5;
+#   mass / A mass drift     FP16 2.7e-7 / -2.2e-2, BF16 7.3e-6 / -4.6e-2,
+#   FX16 1.4e-7 / 4.1e-4.
+# psi = -1 + 2 rho_A / rho near -1 in the B phase: FP16 steps of 4.9e-4 (BF16
+# 3.9e-3) round the A traces there away each step, FX16's uniform steps
+# (4.6e-5 over +-1.5) with unbiased rounding keep the A mass 50x better.  Its
+# velocity error is the larger one: the droplet's currents (~1e-3) are small
+# against the default u range +-0.375 (steps of 1.1e-5), where FP16's
+# relative resolution is finer; tslb_set_storage_range narrows it.
+STORAGE_TOL = {
+    "fp16": dict(max=(3.5e-5, 6.2e-3, 2.4e-3, 1.0e-3), rms=(4.1e-6, 2.3e-3, 1.9e-3, 3.6e-4), mass=4.1e-7,
+                 amass=3.3e-2),
+    "bf16": dict(max=(2.8e-4, 4.0e-2, 9.6e-3, 1.0e-2), rms=(2.4e-5, 1.3e-2, 1.4e-2, 3.4e-3), mass=1.1e-5,
+                 amass=6.9e-2),
+    "fx16": dict(max=(2.6e-5, 1.3e-2, 1.9e-3, 1.2e-4), rms=(6.8e-6, 9.1e-3, 3.2e-3, 3.9e-5), mass=2.1e-7,
+                 amass=6.1e-4),
+}
+GROUPS = (("rho",), ("u", "v", "w"), ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"), ("phi",))
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16", "fx16"])
+def test_colour_reduced_storage(storage):
+    from parity import field_rms_errors
+    case = configs.colour_droplet(n=24, radius=6.0, tau=0.8, tau_b=0.8)
+    nz, ny, nx = case.shape
+    f = case.host_fields()
+    with Lattice(nx, ny, nz, storage=storage, **case.lattice) as lat:
+        for name in ("rho", "u", "v", "w", "phi"):
+            lat.upload_field(name, f[name])
+        lat.init_equilibrium(False)
+        lat.step(15)
+        got = lat.download()
+    ref = oracle_run(case, 15)
+    e, r = field_errors(got, ref), field_rms_errors(got, ref)
+    emax = [max(e[k] for k in g) for g in GROUPS]
+    erms = [max(r[k] for k in g) for g in GROUPS]
+    print(storage, "colour droplet 24^3 vs FP64 max", ["%.2e" % v for v in emax], "rms", ["%.2e" % v for v in erms])
+    assert np.isfinite(got).all()
+    tol = STORAGE_TOL[storage]
+    for v, t in zip(emax, tol["max"]):
+        assert v <= t
+    for v, t in zip(erms, tol["rms"]):
+        assert v <= t
+    n, R, sigma = 32, 8.0, 0.02
+    case = configs.colour_droplet(n=n, radius=R, sigma=sigma, tau=1.0, tau_b=1.0, noise=0.0)
+    with Lattice(n, n, n, storage=storage, **case.lattice) as lat:
+        case.apply(lat)
+        r0 = lat.reduce()
+        lat.step(500)
+        r1 = lat.reduce()
+        m = lat.download()
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    rr = np.sqrt((k + 1 - n / 2) ** 2 + (j + 1 - n / 2) ** 2 + (i + 1 - n / 2) ** 2)
+    dp = (m[0][rr < R - 3].mean() - m[0][rr > R + 5].mean()) / 3.0
+    dm, da = (r1[0] - r0[0]) / r0[0], (r1[1] - r0[1]) / r0[1]
+    print(storage, "colour 32^3 500 steps: Laplace", dp / (2 * sigma / R), "mass drift", dm, "A mass drift", da)
+    assert np.isfinite(m).all()
+    assert abs(dp / (2 * sigma / R) - 1) < 0.1
+    assert abs(dm) <= tol["mass"]
+    assert abs(da) <= tol["amass"]
