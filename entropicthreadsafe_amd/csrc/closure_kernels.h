@@ -482,7 +482,7 @@ struct StateView {
   __device__ __forceinline__ float get(int q, long long id, const Params &p) const {
     const float v = to_f(base[q * fs + id]);
     if constexpr (sizeof(S) == 4) return v;
-    else return v * p.isc[q];
+    else return v * p.isc[sgrp(q)];
   }
 };
 struct PlainView {

@@ -137,8 +137,8 @@ struct Params {
   float nua, nub;           // colour model: viscosities of components A and B
   float beta;               // colour model: recolouring parameter
   float tau;                // relaxation time (learned closure: omega = 1/(tau + 3 nu_t))
-  float sc[TSLB_NFIELDS];   // storage scale per field (1 for FP32)
-  float isc[TSLB_NFIELDS];  // decode scale: 1 / sc (FX16: 1 / (32767 sc))
+  float sc[4];   // storage scale per field group, sgrp(q) (1 for FP32)
+  float isc[4];  // decode scale: 1 / sc (FX16: 1 / (32767 sc))
   uint32_t step;            // fused launches so far (FX16: the dither of the stored rho - 1 and phi)
   unsigned int *sat;        // FX16: count of stored values clipped at their range (device)
 };
@@ -153,6 +153,12 @@ struct Params {
 // Its scale is sc = 1 / R_q and the encode is v_cvt_pknorm_i16_f32 of v sc
 // (clamp to [-1, 1], times 32767, round to nearest: one instruction for two
 // values, saturated at +-R_q); isc = R_q / 32767 decodes.
+// Storage scales are shared by field groups: rho - 1; u, v, w; the six
+// Pi^neq; phi (psi).  Four scales instead of eleven keep them in scalar
+// registers through the fused pass (the per-field form spilled them into
+// vector lanes and reloaded them every plane).
+__host__ __device__ constexpr int sgrp(int q) { return q == 0 ? 0 : (q < 4 ? 1 : (q < 10 ? 2 : 3)); }
+
 struct bf16_t {
   uint16_t b;
 };
@@ -162,7 +168,15 @@ struct fx16_t {
 __device__ __forceinline__ float to_f(float x) { return x; }
 __device__ __forceinline__ float to_f(_Float16 x) { return (float)x; }
 __device__ __forceinline__ float to_f(bf16_t x) { return __uint_as_float((uint32_t)x.b << 16); }
-__device__ __forceinline__ float to_f(fx16_t x) { return (float)x.v; }
+// FX16 decode without a sign extension: the 16 bits with the sign bit
+// flipped are v + 32768, placed in the mantissa of 2^23 and the offset
+// subtracted (exact).  A sign extension would be folded into the load
+// (buffer_load_sshort) at the point of use, and the raw loads of one source
+// would then be issued one at a time, each waited out before the next.
+__device__ __forceinline__ float to_f(fx16_t x) {
+  const uint32_t u = (uint16_t)x.v;
+  return __uint_as_float((u ^ 0x8000u) | 0x4B000000u) - 8421376.f;
+}
 template <typename S>
 __device__ __forceinline__ S from_f(float v);
 template <>
@@ -208,7 +222,7 @@ struct CFieldsT {
 template <typename S>
 __device__ __forceinline__ float ldf(const S *__restrict__ a, int id, const Params &p, int q) {
   if constexpr (sizeof(S) == 4) return to_f(a[id]);
-  else return to_f(a[id]) * p.isc[q];
+  else return to_f(a[id]) * p.isc[sgrp(q)];
 }
 template <typename S>
 __device__ __forceinline__ void stf(S *__restrict__ a, int id, float v, const Params &p, int q) {
@@ -216,9 +230,9 @@ __device__ __forceinline__ void stf(S *__restrict__ a, int id, float v, const Pa
     a[id] = from_f<S>(v);
   } else {
     if constexpr (std::is_same<S, fx16_t>::value) {
-      if (!(fabsf(v * p.sc[q]) <= 1.f) && p.sat) atomicAdd(p.sat, 1u);
+      if (!(fabsf(v * p.sc[sgrp(q)]) <= 1.f) && p.sat) atomicAdd(p.sat, 1u);
     }
-    a[id] = from_f<S>(v * p.sc[q]);
+    a[id] = from_f<S>(v * p.sc[sgrp(q)]);
   }
 }
 
@@ -679,7 +693,7 @@ __device__ __forceinline__ void load_raw(const PlaneSrc &ps, uint32_t offS, RawS
 template <int NC, typename S, bool CLO>
 __device__ __forceinline__ void convert_raw(const RawSrc<NC, S> &r, const Params &p, Src<NC, CLO> &s) {
 #pragma unroll
-  for (int q = 0; q < 10; q++) s.m[q] = to_f(r.v[q]) * p.isc[q];
+  for (int q = 0; q < 10; q++) s.m[q] = to_f(r.v[q]) * p.isc[sgrp(q)];
 }
 
 // FP32 storage: the moments of one source straight into place.
@@ -1099,8 +1113,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       d[tid] = to_f(pr0);
       if (ptwo) d[tid + NTHR] = to_f(pr1);
     } else {
-      d[tid] = to_f(pr0) * p.isc[TSLB_PHI];
-      if (ptwo) d[tid + NTHR] = to_f(pr1) * p.isc[TSLB_PHI];
+      d[tid] = to_f(pr0) * p.isc[sgrp(TSLB_PHI)];
+      if (ptwo) d[tid + NTHR] = to_f(pr1) * p.isc[sgrp(TSLB_PHI)];
     }
   };
   // Phase field: the normals of both sources come from the separated
@@ -1358,7 +1372,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
               bld<S>(rsrc((const char *)src + zb + q * fsS, pbS), offS));
         if constexpr (NC == 2) {
           float f = to_f(bld<S>(rsrc((const char *)src + zb + TSLB_PHI * fsS, pbS), offS));
-          if constexpr (sizeof(S) != 4) f *= p.isc[TSLB_PHI];
+          if constexpr (sizeof(S) != 4) f *= p.isc[sgrp(TSLB_PHI)];
           hit = f * (1.f - f) > 0.19f;
         }
       } else {
@@ -1387,7 +1401,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         if constexpr (std::is_same<S, fx16_t>::value) {
           float x[NF];
 #pragma unroll
-          for (int q = 0; q < NF; q++) x[q] = out[q] * p.sc[q];
+          for (int q = 0; q < NF; q++) x[q] = out[q] * p.sc[sgrp(q)];
           // count the nodes with a value beyond its field's range (clipped)
           const uint32_t hd = fx16_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step * 0x632BE5ABu);
           // One component: round, clamp and convert per field, rho - 1 as
@@ -1423,7 +1437,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           for (int q = 0; q < NF; q++) {
             const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
             if constexpr (sizeof(S) == 4) bst<2>(r, offS, from_f<S>(out[q]));
-            else bst<2>(r, offS, from_f<S>(out[q] * p.sc[q]));
+            else bst<2>(r, offS, from_f<S>(out[q] * p.sc[sgrp(q)]));
           }
         }
         if constexpr (NC == 2) hit = out[10] * (1.f - out[10]) > 0.19f;
@@ -1555,8 +1569,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
     pr0 = to_f(q.a);
     pr1 = two ? to_f(q.b) : 0.f;
     if constexpr (sizeof(S) != 4) {
-      pr0 *= p.isc[TSLB_PHI];
-      pr1 *= p.isc[TSLB_PHI];
+      pr0 *= p.isc[sgrp(TSLB_PHI)];
+      pr1 *= p.isc[sgrp(TSLB_PHI)];
     }
   };
   // store the loaded plane z and post this wave's votes: are all its values
@@ -1779,7 +1793,7 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
   S q0 = ld(z0), q1 = ld(z0 + 1), q2 = ld(z0 + 2), q3 = ld(z0 + 3);
   auto body = [&](int z, S &q) {
     float f = to_f(q);
-    if constexpr (sizeof(S) != 4) f *= p.isc[TSLB_PHI];
+    if constexpr (sizeof(S) != 4) f *= p.isc[sgrp(TSLB_PHI)];
     q = ld(z + 4);
     const bool hit = active && f * (1.f - f) > 0.19f;
     const uint64_t bal = __ballot(hit);
@@ -1857,8 +1871,8 @@ __global__ __launch_bounds__(HNT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     auto commit_from = [&](int z, S ra, S rb) {
       float a = to_f(ra), c = two ? to_f(rb) : 0.f;
       if constexpr (sizeof(S) != 4) {
-        a *= p.isc[TSLB_PHI];
-        c *= p.isc[TSLB_PHI];
+        a *= p.isc[sgrp(TSLB_PHI)];
+        c *= p.isc[sgrp(TSLB_PHI)];
       }
       ph[rph(z)][tid] = a;
       if (two) ph[rph(z)][tid + HNT] = c;
@@ -2432,17 +2446,17 @@ namespace {
 // colour psi) within +-1.5 (a sharp droplet's start-up overshoots phi to
 // ~1.15).  Resolution = range / 32767.  Clipped values
 // are counted (tslb_storage_clipped).
-// the encode scale sc of field q and its decode isc: FX16 encodes v sc in
-// [-1, 1] as snorm16 (32767 steps), the other formats store v sc
-void set_scale(Params &k, int storage, int q, float sc) {
-  k.sc[q] = sc;
-  k.isc[q] = storage == TSLB_STORAGE_FX16 ? 1.f / (32767.f * sc) : 1.f / sc;
+// the encode scale sc of field group g and its decode isc: FX16 encodes v sc
+// in [-1, 1] as snorm16 (32767 steps), the other formats store v sc
+void set_scale(Params &k, int storage, int g, float sc) {
+  k.sc[g] = sc;
+  k.isc[g] = storage == TSLB_STORAGE_FX16 ? 1.f / (32767.f * sc) : 1.f / sc;
 }
 
-float fx16_range(int q) {
-  if (q == TSLB_RHO) return 0.25f;
-  if (q >= TSLB_U && q <= TSLB_W) return 0.375f;
-  if (q >= TSLB_PXX && q <= TSLB_PYZ) return 0.125f;
+float fx16_range(int g) {
+  if (g == sgrp(TSLB_RHO)) return 0.25f;
+  if (g == sgrp(TSLB_U)) return 0.375f;
+  if (g == sgrp(TSLB_PXX)) return 0.125f;
   return 1.5f;
 }
 
@@ -2821,19 +2835,19 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
     k.z_chunk = zc;
   }
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
-  // storage scales: FP16 keeps rho-1 (x64) and Pi^neq (x4096) in its normal
-  // range; BF16 has FP32's exponent range and FP32 needs none; FX16 maps each
-  // field's half range onto 32767 (default ranges in fx16_range, changed by
-  // tslb_set_storage_range)
+  // storage scales per field group: FP16 keeps rho-1 (x64) and Pi^neq
+  // (x4096) in its normal range; BF16 has FP32's exponent range and FP32
+  // needs none; FX16 maps each group's half range onto 32767 (default ranges
+  // in fx16_range, changed by tslb_set_storage_range)
   c->es = p->storage == TSLB_STORAGE_FP32 ? 4 : 2;
-  for (int q = 0; q < TSLB_NFIELDS; q++) {
+  for (int g = 0; g < 4; g++) {
     float sc = 1.f;
     if (p->storage == TSLB_STORAGE_FP16) {
-      if (q == TSLB_RHO) sc = 64.f;
-      if (q >= TSLB_PXX && q <= TSLB_PYZ) sc = 4096.f;
+      if (g == sgrp(TSLB_RHO)) sc = 64.f;
+      if (g == sgrp(TSLB_PXX)) sc = 4096.f;
     }
-    if (p->storage == TSLB_STORAGE_FX16) sc = 1.f / fx16_range(q);
-    set_scale(k, p->storage, q, sc);
+    if (p->storage == TSLB_STORAGE_FX16) sc = 1.f / fx16_range(g);
+    set_scale(k, p->storage, g, sc);
   }
 
   if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 7;
@@ -3017,8 +3031,8 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   CkptHeader h;
   memset(&h, 0, sizeof(h));
   memcpy(h.magic, "TSLBCKPT", 8);
-  // version 2: FX16 storage, whose 11 field scales follow the header
-  h.version = c->prm.storage == TSLB_STORAGE_FX16 ? 2 : 1;
+  // version 3: FX16 storage, whose 4 group scales and step count follow the header
+  h.version = c->prm.storage == TSLB_STORAGE_FX16 ? 3 : 1;
   h.nx = c->prm.nx;
   h.ny = c->prm.ny;
   h.nz = c->prm.nz;
@@ -3032,8 +3046,8 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   const size_t bytes = c->n * c->es;
   std::vector<char> host(bytes);
   bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
-  if (ok && h.version == 2) ok = fwrite(c->kp.sc, sizeof(float), TSLB_NFIELDS, fp) == (size_t)TSLB_NFIELDS;
-  if (ok && h.version == 2) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
+  if (ok && h.version == 3) ok = fwrite(c->kp.sc, sizeof(float), 4, fp) == 4;
+  if (ok && h.version == 3) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
   for (int q = 0; ok && q < TSLB_NFIELDS; q++) {
     hipError_t e = hipMemcpyAsync(host.data(), (char *)c->buf[c->cur] + (size_t)q * c->fs * c->es, bytes,
                                   hipMemcpyDeviceToHost, c->stream);
@@ -3057,7 +3071,7 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
   if (!fp) return fail(TSLB_E_ARG, std::string("cannot open ") + path);
   CkptHeader h;
   if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0 ||
-      h.version != (h.storage == TSLB_STORAGE_FX16 ? 2 : 1)) {
+      h.version != (h.storage == TSLB_STORAGE_FX16 ? 3 : 1)) {
     fclose(fp);
     return fail(TSLB_E_ARG, std::string(path) + " is not a TSLB checkpoint");
   }
@@ -3066,21 +3080,20 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
     fclose(fp);
     return fail(TSLB_E_ARG, "checkpoint lattice, components or storage differ from the context");
   }
-  float sc[TSLB_NFIELDS];
+  float sc[4];
   uint32_t kstep = 0;
-  if (h.version == 2 && (fread(sc, sizeof(float), TSLB_NFIELDS, fp) != (size_t)TSLB_NFIELDS ||
-                         fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
+  if (h.version == 3 && (fread(sc, sizeof(float), 4, fp) != 4 || fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
     fclose(fp);
     return fail(TSLB_E_ARG, std::string("truncated checkpoint ") + path);
   }
-  if (h.version == 2) {
-    for (int q = 0; q < TSLB_NFIELDS; q++) {
-      if (!(sc[q] > 0.f) || !(sc[q] < 1e30f)) {
+  if (h.version == 3) {
+    for (int g = 0; g < 4; g++) {
+      if (!(sc[g] > 0.f) || !(sc[g] < 1e30f)) {
         fclose(fp);
         return fail(TSLB_E_ARG, std::string(path) + " holds an invalid FX16 scale");
       }
     }
-    for (int q = 0; q < TSLB_NFIELDS; q++) set_scale(c->kp, TSLB_STORAGE_FX16, q, sc[q]);
+    for (int g = 0; g < 4; g++) set_scale(c->kp, TSLB_STORAGE_FX16, g, sc[g]);
     c->kp.step = kstep;
   }
   const size_t bytes = c->n * c->es;
@@ -3250,17 +3263,22 @@ int tslb_set_storage_range(tslb_ctx *c, int field, float half_range) {
   if (c->prm.storage != TSLB_STORAGE_FX16) return fail(TSLB_E_STATE, "storage ranges apply to TSLB_STORAGE_FX16");
   if (field < 0 || field >= TSLB_NFIELDS) return fail(TSLB_E_ARG, "bad field");
   if (!(half_range > 0.f) || !(half_range < 1e30f)) return fail(TSLB_E_ARG, "half_range must be positive and finite");
-  // re-encode the stored field: decode with the old scale into the second
-  // buffer (scratch between steps), encode with the new one
+  // the range belongs to the field's group; re-encode each stored field of
+  // the group: decode with the old scale into the second buffer (scratch
+  // between steps), encode with the new one
+  const Params old = c->kp;
+  set_scale(c->kp, TSLB_STORAGE_FX16, sgrp(field), 1.f / half_range);
   float *tmp = (float *)c->buf[1 - c->cur];
-  fx16_t *f = field_ptr<fx16_t>(c->buf[c->cur], c->fs, field);
   const dim3 g((unsigned)((c->n + 255) / 256));
-  hipLaunchKernelGGL(k_from_storage<fx16_t>, g, dim3(256), 0, c->stream, (const fx16_t *)f, tmp, (int)c->n, field,
-                     c->kp);
-  HIPCK(hipGetLastError());
-  set_scale(c->kp, TSLB_STORAGE_FX16, field, 1.f / half_range);
-  hipLaunchKernelGGL(k_to_storage<fx16_t>, g, dim3(256), 0, c->stream, (const float *)tmp, f, (int)c->n, field, c->kp);
-  HIPCK(hipGetLastError());
+  for (int q = 0; q < TSLB_NFIELDS; q++) {
+    if (sgrp(q) != sgrp(field)) continue;
+    fx16_t *f = field_ptr<fx16_t>(c->buf[c->cur], c->fs, q);
+    hipLaunchKernelGGL(k_from_storage<fx16_t>, g, dim3(256), 0, c->stream, (const fx16_t *)f, tmp, (int)c->n, q,
+                       old);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_to_storage<fx16_t>, g, dim3(256), 0, c->stream, (const float *)tmp, f, (int)c->n, q, c->kp);
+    HIPCK(hipGetLastError());
+  }
   HIPCK(hipStreamSynchronize(c->stream));
   c->normals_fresh = 0;
   return TSLB_OK;
