@@ -1402,36 +1402,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           float x[NF];
 #pragma unroll
           for (int q = 0; q < NF; q++) x[q] = out[q] * p.sc[sgrp(q)];
-          // count the nodes with a value beyond its field's range (clipped)
           const uint32_t hd = fx16_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step * 0x632BE5ABu);
-          // One component: round, clamp and convert per field, rho - 1 as
-          // floor(32767 x + r) with r uniform in [0, 1) (the same rounding as
-          // below: this form measured 0.90 ms per channel launch against 1.13
-          // ms with v_cvt_pknorm, which the phase-field kernel runs 5 %
-          // faster with: 2.83 against 2.97 ms on the 512^3 emulsion)
-          if constexpr (NC == 1) {
-            bool clip = false;
+          // count the nodes with a value beyond its field's range (clipped)
+          float mx = 0.f;
 #pragma unroll
-            for (int q = 0; q < NF; q++) clip |= !(fabsf(x[q]) <= 1.f);
-            if (clip && p.sat) atomicAdd(p.sat, 1u);
-            const float r = (float)(hd >> 8) * (1.f / 16777216.f);
+          for (int q = 0; q < NF; q++) mx = fmaxf(mx, fabsf(x[q]));
+          if (mx > 1.f && p.sat) atomicAdd(p.sat, 1u);
+          // the conserved fields with unbiased rounding
+          x[0] += fx16_offset(hd >> 9);
+          if constexpr (NF > TSLB_PHI) x[TSLB_PHI] += fx16_offset(hd & 0x7FFFFFu);
 #pragma unroll
-            for (int q = 0; q < NF; q++) {
-              const float v = x[q] * 32767.f;
-              const float w = q == 0 ? floorf(v + r) : rintf(v);
-              bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS, fx16_t{(int16_t)fminf(fmaxf(w, -32767.f), 32767.f)});
-            }
-          } else {
-            float mx = 0.f;
-#pragma unroll
-            for (int q = 0; q < NF; q++) mx = fmaxf(mx, fabsf(x[q]));
-            if (mx > 1.f && p.sat) atomicAdd(p.sat, 1u);
-            // the conserved fields with unbiased rounding
-            x[0] += fx16_offset(hd >> 9);
-            if constexpr (NF > TSLB_PHI) x[TSLB_PHI] += fx16_offset(hd & 0x7FFFFFu);
-#pragma unroll
-            for (int q = 0; q < NF; q++) bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS, from_f<S>(x[q]));
-          }
+          for (int q = 0; q < NF; q++) bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS, from_f<S>(x[q]));
         } else {
 #pragma unroll
           for (int q = 0; q < NF; q++) {
