@@ -480,9 +480,7 @@ struct StateView {
   const S *base;
   long long fs;
   __device__ __forceinline__ float get(int q, long long id, const Params &p) const {
-    const float v = to_f(base[q * fs + id]);
-    if constexpr (sizeof(S) == 4) return v;
-    else return v * p.isc[sgrp(q)];
+    return dec(base[q * fs + id], p, sgrp(q));
   }
 };
 struct PlainView {

@@ -138,7 +138,7 @@ struct Params {
   float beta;               // colour model: recolouring parameter
   float tau;                // relaxation time (learned closure: omega = 1/(tau + 3 nu_t))
   float sc[4];   // storage scale per field group, sgrp(q) (1 for FP32)
-  float isc[4];  // decode scale: 1 / sc (FX16: 1 / (32767 sc))
+  float isc[4];  // decode scale: 1 / sc
   uint32_t step;            // fused launches so far (FX16: the dither of the stored rho - 1 and phi)
   unsigned int *sat;        // FX16: count of stored values clipped at their range (device)
 };
@@ -148,11 +148,14 @@ struct Params {
 // storage keeps every field pre-multiplied by a per-field scale (Params::sc)
 // so that the shifted density and Pi^neq sit in FP16's normal range; compute
 // is always FP32.  FX16 stores each field as a 16-bit signed fixed-point
-// number round(32767 v / R_q) (R_q the field's half range): uniform absolute
-// resolution R_q / 32767 instead of 11 (FP16) or 8 (BF16) significant bits.
-// Its scale is sc = 1 / R_q and the encode is v_cvt_pknorm_i16_f32 of v sc
-// (clamp to [-1, 1], times 32767, round to nearest: one instruction for two
-// values, saturated at +-R_q); isc = R_q / 32767 decodes.
+// number k = round(v 2^m), |k| <= 32767: uniform absolute resolution 2^-m (m per field group) instead of 11 (FP16)
+// or 8 (BF16) significant bits.  The scales sc = 2^m and isc = 2^-m are
+// powers of two, so decode and encode are exact inverses: a value that the
+// step leaves unchanged is stored again as the same k, also under the
+// stochastic rounding below.  (With a scale such as 32767 / R the product
+// isc sc 32767 differs from 1 by ~1e-7, and unbiased rounding carried that
+// bias into every step: the phase mass of the 512^3 emulsion drifted by
+// 1e-3 over 20 000 steps.)
 // Storage scales are shared by field groups: rho - 1; u, v, w; the six
 // Pi^neq; phi (psi).  Four scales instead of eleven keep them in scalar
 // registers through the fused pass (the per-field form spilled them into
@@ -168,14 +171,17 @@ struct fx16_t {
 __device__ __forceinline__ float to_f(float x) { return x; }
 __device__ __forceinline__ float to_f(_Float16 x) { return (float)x; }
 __device__ __forceinline__ float to_f(bf16_t x) { return __uint_as_float((uint32_t)x.b << 16); }
-// FX16 decode without a sign extension: the 16 bits with the sign bit
-// flipped are v + 32768, placed in the mantissa of 2^23 and the offset
-// subtracted (exact).  A sign extension would be folded into the load
-// (buffer_load_sshort) at the point of use, and the raw loads of one source
-// would then be issued one at a time, each waited out before the next.
+// FX16 decode: one convert that sign-extends the low 16 bits of the
+// zero-extended load (SDWA).  Written out because the compiler folds a sign
+// extension into the load (buffer_load_sshort) at its point of use, and the
+// raw loads of one source were then issued one at a time, each waited out
+// before the next.  Code 0 is 0, so a load past the buffer's end reads 0.
 __device__ __forceinline__ float to_f(fx16_t x) {
-  const uint32_t u = (uint16_t)x.v;
-  return __uint_as_float((u ^ 0x8000u) | 0x4B000000u) - 8421376.f;
+  float f;
+  asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_0"
+      : "=v"(f)
+      : "v"((uint32_t)(uint16_t)x.v));
+  return f;
 }
 template <typename S>
 __device__ __forceinline__ S from_f(float v);
@@ -189,24 +195,56 @@ __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return bf16_t{(uint16_t)(u >> 16)};
 }
+// FX16 encode by the magic number C = 1.5 2^23: for |y| < 2^22 the float
+// y + C lies in [2^23, 2^24), where the spacing is 1, so the one rounding of
+// the sum is round-to-nearest-even of y, and its low 16 bits are k in two's
+// complement.  No clamp: |y| < 32767.5 is the caller's to ensure.
+constexpr float kFx16Magic = 12582912.f;
+__device__ __forceinline__ fx16_t fx16_code(float t) {
+  return fx16_t{(int16_t)(uint16_t)__float_as_uint(t + kFx16Magic)};
+}
+// round to nearest, clamped to +-32767
 template <>
 __device__ __forceinline__ fx16_t from_f<fx16_t>(float v) {
-  return fx16_t{__builtin_amdgcn_cvt_pknorm_i16(v, 0.f).x};
+  return fx16_code(__builtin_amdgcn_fmed3f(v, -32767.f, 32767.f));
 }
-// FX16 with unbiased (stochastic) rounding: round-to-nearest of 32767 v sc
-// + d, d uniform in [-1/2, 1/2), is floor(32767 v sc + d + 1/2), whose mean
-// is 32767 v sc.  Round-to-nearest alone is biased wherever a field is nearly
-// uniform (every node of a resting phase rounds the same way), which drifts
-// the conserved sums; the fused pass stores rho - 1 and phi this way, so
-// they drift as a random walk.  One multiplicative hash of the node and a
-// per-step seed gives both offsets, 23 bits each: d = m - 3/2 with m in
-// [1, 2) the float whose mantissa holds the bits.
+// FX16 with unbiased (stochastic) rounding: round-to-nearest of y + e with e
+// uniform in (-1/2, 1/2) takes the upper neighbour of y with probability
+// frac(y), so its mean is y.  Round-to-nearest alone is biased wherever a
+// field is nearly uniform (every node of a resting phase rounds the same
+// way), which drifts the conserved sums, and it drops every per-step change
+// below half a step (a slowly forced flow stagnates); the fused pass stores
+// rho - 1, u, v, w and phi this way.  e = (w + 1/2) 2^-16 - 1/2 for a 16-bit
+// word w of a hash of the node and the step; the caller passes
+// yh = y - 1/2 + 2^-17 (folded into the fma that scales the value).
+constexpr float kFx16DitherShift = -0.5f + 0x1p-17f;
+__device__ __forceinline__ fx16_t fx16_dither(float yh, uint32_t w16) {
+  return fx16_code(fmaf((float)w16, 0x1p-16f, yh));
+}
+// The hash words must average to uniform over the lattice step by step:
+// with the seed XORed into the node index, the set of hashed values over a
+// power-of-two lattice is the same every step, and the small deviation of
+// its mean from uniform (~1e-5) repeated every step (a phase-mass drift of
+// 1e-3 over 20 000 steps of the 512^3 emulsion).  Adding the seed moves the
+// window of hashed values instead, so the deviation changes sign and size
+// from step to step and averages out.
 __device__ __forceinline__ uint32_t fx16_hash(uint32_t node, uint32_t seed) {
-  const uint32_t h = (node ^ seed) * 0x9E3779B1u;
+  const uint32_t h = (node + seed) * 0x9E3779B1u;
   return h ^ (h >> 16);
 }
-__device__ __forceinline__ float fx16_offset(uint32_t bits23) {
-  return (__uint_as_float(0x3F800000u | bits23) - 1.5f) * (1.f / 32767.f);
+__device__ __forceinline__ uint32_t fx16_rehash(uint32_t h) {
+  h *= 0x85EBCA6Bu;
+  return h ^ (h >> 13);
+}
+// the fields stored with unbiased rounding and the hash word each takes
+__host__ __device__ constexpr bool fx16_dithered(int q) { return q < 4 || q == TSLB_PHI; }
+__host__ __device__ constexpr int fx16_word(int q) { return q < 4 ? q : 4; }
+
+// a stored value of field group g, decoded
+template <typename S>
+__device__ __forceinline__ float dec(S x, const Params &p, int g) {
+  if constexpr (sizeof(S) == 4) return to_f(x);
+  else return to_f(x) * p.isc[g];
 }
 
 template <typename S>
@@ -221,8 +259,7 @@ struct CFieldsT {
 // scaled field access
 template <typename S>
 __device__ __forceinline__ float ldf(const S *__restrict__ a, int id, const Params &p, int q) {
-  if constexpr (sizeof(S) == 4) return to_f(a[id]);
-  else return to_f(a[id]) * p.isc[sgrp(q)];
+  return dec(a[id], p, sgrp(q));
 }
 template <typename S>
 __device__ __forceinline__ void stf(S *__restrict__ a, int id, float v, const Params &p, int q) {
@@ -230,7 +267,7 @@ __device__ __forceinline__ void stf(S *__restrict__ a, int id, float v, const Pa
     a[id] = from_f<S>(v);
   } else {
     if constexpr (std::is_same<S, fx16_t>::value) {
-      if (!(fabsf(v * p.sc[sgrp(q)]) <= 1.f) && p.sat) atomicAdd(p.sat, 1u);
+      if (!(fabsf(v * p.sc[sgrp(q)]) <= 32767.5f) && p.sat) atomicAdd(p.sat, 1u);
     }
     a[id] = from_f<S>(v * p.sc[sgrp(q)]);
   }
@@ -693,7 +730,7 @@ __device__ __forceinline__ void load_raw(const PlaneSrc &ps, uint32_t offS, RawS
 template <int NC, typename S, bool CLO>
 __device__ __forceinline__ void convert_raw(const RawSrc<NC, S> &r, const Params &p, Src<NC, CLO> &s) {
 #pragma unroll
-  for (int q = 0; q < 10; q++) s.m[q] = to_f(r.v[q]) * p.isc[sgrp(q)];
+  for (int q = 0; q < 10; q++) s.m[q] = dec(r.v[q], p, sgrp(q));
 }
 
 // FP32 storage: the moments of one source straight into place.
@@ -1109,13 +1146,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     if (ptwo) pr1 = bld<S>(r, pofs1);
   };
   auto phi_commit = [&](float *d) {
-    if constexpr (sizeof(S) == 4) {
-      d[tid] = to_f(pr0);
-      if (ptwo) d[tid + NTHR] = to_f(pr1);
-    } else {
-      d[tid] = to_f(pr0) * p.isc[sgrp(TSLB_PHI)];
-      if (ptwo) d[tid + NTHR] = to_f(pr1) * p.isc[sgrp(TSLB_PHI)];
-    }
+    d[tid] = dec(pr0, p, sgrp(TSLB_PHI));
+    if (ptwo) d[tid + NTHR] = dec(pr1, p, sgrp(TSLB_PHI));
   };
   // Phase field: the normals of both sources come from the separated
   // gradient stencil, marched along z with the source planes (footprint
@@ -1371,8 +1403,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS,
               bld<S>(rsrc((const char *)src + zb + q * fsS, pbS), offS));
         if constexpr (NC == 2) {
-          float f = to_f(bld<S>(rsrc((const char *)src + zb + TSLB_PHI * fsS, pbS), offS));
-          if constexpr (sizeof(S) != 4) f *= p.isc[sgrp(TSLB_PHI)];
+          const float f = dec(bld<S>(rsrc((const char *)src + zb + TSLB_PHI * fsS, pbS), offS), p, sgrp(TSLB_PHI));
           hit = f * (1.f - f) > 0.19f;
         }
       } else {
@@ -1399,20 +1430,36 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
                                aA.syz - rho * v * w,
                                NC == 3 ? fmaf(2.f * aA.phi, inv, -1.f) : aA.phi};
         if constexpr (std::is_same<S, fx16_t>::value) {
+          // the value in steps, y = v 2^m (yh = y - 1/2 + 2^-17 where it is
+          // dithered)
           float x[NF];
 #pragma unroll
-          for (int q = 0; q < NF; q++) x[q] = out[q] * p.sc[sgrp(q)];
-          const uint32_t hd = fx16_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step * 0x632BE5ABu);
-          // count the nodes with a value beyond its field's range (clipped)
+          for (int q = 0; q < NF; q++)
+            x[q] = fx16_dithered(q) ? fmaf(out[q], p.sc[sgrp(q)], kFx16DitherShift) : out[q] * p.sc[sgrp(q)];
+          // Past |x| = 32766 the values are clamped so that neither rounding
+          // leaves +-32767 (a sum rounded to 32767.5 would round to the even
+          // 32768); a node with a value beyond its field's range, |y| >
+          // 32767.5, is counted.
           float mx = 0.f;
 #pragma unroll
           for (int q = 0; q < NF; q++) mx = fmaxf(mx, fabsf(x[q]));
-          if (mx > 1.f && p.sat) atomicAdd(p.sat, 1u);
-          // the conserved fields with unbiased rounding
-          x[0] += fx16_offset(hd >> 9);
-          if constexpr (NF > TSLB_PHI) x[TSLB_PHI] += fx16_offset(hd & 0x7FFFFFu);
+          if (mx > 32766.f) {
+            bool clip = false;
 #pragma unroll
-          for (int q = 0; q < NF; q++) bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS, from_f<S>(x[q]));
+            for (int q = 0; q < NF; q++)
+              clip |= fabsf(fx16_dithered(q) ? x[q] - kFx16DitherShift : x[q]) > 32767.5f;
+            if (clip && p.sat) atomicAdd(p.sat, 1u);
+#pragma unroll
+            for (int q = 0; q < NF; q++)
+              x[q] = __builtin_amdgcn_fmed3f(x[q], -32767.f, fx16_dithered(q) ? 32766.f : 32767.f);
+          }
+          const uint32_t h1 = fx16_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step * 0x632BE5ABu);
+          const uint32_t h2 = fx16_rehash(h1);
+          const uint32_t w[5] = {h1 & 0xFFFFu, h1 >> 16, h2 & 0xFFFFu, h2 >> 16, (h1 ^ h2) & 0xFFFFu};
+#pragma unroll
+          for (int q = 0; q < NF; q++)
+            bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS,
+                   fx16_dithered(q) ? fx16_dither(x[q], w[fx16_word(q)]) : fx16_code(x[q]));
         } else {
 #pragma unroll
           for (int q = 0; q < NF; q++) {
@@ -1547,12 +1594,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
   };
   float pr0 = 0.f, pr1 = 0.f;
   auto take = [&](const PhiQ &q) {
-    pr0 = to_f(q.a);
-    pr1 = two ? to_f(q.b) : 0.f;
-    if constexpr (sizeof(S) != 4) {
-      pr0 *= p.isc[sgrp(TSLB_PHI)];
-      pr1 *= p.isc[sgrp(TSLB_PHI)];
-    }
+    pr0 = dec(q.a, p, sgrp(TSLB_PHI));
+    pr1 = two ? dec(q.b, p, sgrp(TSLB_PHI)) : 0.f;
   };
   // store the loaded plane z and post this wave's votes: are all its values
   // equal, and may one of the tile's own nodes carry curvature?  The gate
@@ -1773,8 +1816,7 @@ __global__ __launch_bounds__(NTHR) void k_iface_list(const S *__restrict__ phi, 
   uint64_t mask0 = 0, mask1 = 0;  // wave-uniform, per half-tile: bit k for plane z0 + k
   S q0 = ld(z0), q1 = ld(z0 + 1), q2 = ld(z0 + 2), q3 = ld(z0 + 3);
   auto body = [&](int z, S &q) {
-    float f = to_f(q);
-    if constexpr (sizeof(S) != 4) f *= p.isc[sgrp(TSLB_PHI)];
+    const float f = dec(q, p, sgrp(TSLB_PHI));
     q = ld(z + 4);
     const bool hit = active && f * (1.f - f) > 0.19f;
     const uint64_t bal = __ballot(hit);
@@ -1850,11 +1892,7 @@ __global__ __launch_bounds__(HNT) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     };
     auto issue = [&](int z) { issue_to(z, qa, qb); };
     auto commit_from = [&](int z, S ra, S rb) {
-      float a = to_f(ra), c = two ? to_f(rb) : 0.f;
-      if constexpr (sizeof(S) != 4) {
-        a *= p.isc[sgrp(TSLB_PHI)];
-        c *= p.isc[sgrp(TSLB_PHI)];
-      }
+      const float a = dec(ra, p, sgrp(TSLB_PHI)), c = two ? dec(rb, p, sgrp(TSLB_PHI)) : 0.f;
       ph[rph(z)][tid] = a;
       if (two) ph[rph(z)][tid + HNT] = c;
     };
@@ -2421,24 +2459,29 @@ struct tslb_ctx {
 
 namespace {
 
-// Default half ranges of the FX16 fields: rho - 1 within +-1/4 (the start-up
-// pressure waves of a sharp interface reach ~0.1), velocities within +-0.375
-// (beyond the lattice's stable range), Pi^neq within +-1/8, phi (or the
-// colour psi) within +-1.5 (a sharp droplet's start-up overshoots phi to
-// ~1.15).  Resolution = range / 32767.  Clipped values
-// are counted (tslb_storage_clipped).
-// the encode scale sc of field group g and its decode isc: FX16 encodes v sc
-// in [-1, 1] as snorm16 (32767 steps), the other formats store v sc
-void set_scale(Params &k, int storage, int g, float sc) {
+// Default FX16 steps 2^-m per group, range 32767 2^-m: rho - 1 2^-17 (within
+// +-1/4: the start-up pressure waves of a sharp interface reach ~0.1),
+// velocities 2^-17 (+-1/4, at the lattice's stable limit), Pi^neq 2^-18
+// (+-1/8), phi (or the colour psi) 2^-14 (+-2: a sharp droplet's start-up
+// overshoots phi to ~1.15).  Clipped values are counted
+// (tslb_storage_clipped).
+// the encode scale sc of field group g and its decode isc = 1 / sc (both
+// powers of two for FX16)
+void set_scale(Params &k, int g, float sc) {
   k.sc[g] = sc;
-  k.isc[g] = storage == TSLB_STORAGE_FX16 ? 1.f / (32767.f * sc) : 1.f / sc;
+  k.isc[g] = 1.f / sc;
 }
 
-float fx16_range(int g) {
-  if (g == sgrp(TSLB_RHO)) return 0.25f;
-  if (g == sgrp(TSLB_U)) return 0.375f;
-  if (g == sgrp(TSLB_PXX)) return 0.125f;
-  return 1.5f;
+int fx16_shift(int g) {
+  if (g == sgrp(TSLB_RHO)) return 17;
+  if (g == sgrp(TSLB_U)) return 17;
+  if (g == sgrp(TSLB_PXX)) return 18;
+  return 14;
+}
+// the finest power-of-two step 2^-m whose range 32767 2^-m covers R
+int fx16_shift_for(float half_range) {
+  int m = (int)floorf(log2f(32767.f / half_range));
+  return m < -30 ? -30 : (m > 40 ? 40 : m);
 }
 
 // call fn with a value of the storage type of the context
@@ -2818,8 +2861,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.nzc = (p->nz + k.z_chunk - 1) / k.z_chunk;
   // storage scales per field group: FP16 keeps rho-1 (x64) and Pi^neq
   // (x4096) in its normal range; BF16 has FP32's exponent range and FP32
-  // needs none; FX16 maps each group's half range onto 32767 (default ranges
-  // in fx16_range, changed by tslb_set_storage_range)
+  // needs none; FX16 steps are powers of two per group (defaults in
+  // fx16_shift, changed by tslb_set_storage_range)
   c->es = p->storage == TSLB_STORAGE_FP32 ? 4 : 2;
   for (int g = 0; g < 4; g++) {
     float sc = 1.f;
@@ -2827,8 +2870,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
       if (g == sgrp(TSLB_RHO)) sc = 64.f;
       if (g == sgrp(TSLB_PXX)) sc = 4096.f;
     }
-    if (p->storage == TSLB_STORAGE_FX16) sc = 1.f / fx16_range(g);
-    set_scale(k, p->storage, g, sc);
+    if (p->storage == TSLB_STORAGE_FX16) sc = ldexpf(1.f, fx16_shift(g));
+    set_scale(k, g, sc);
   }
 
   if (const char *v = getenv("TSLB_FUSED_VARIANT")) c->variant = atoi(v) & 7;
@@ -3005,6 +3048,12 @@ struct CkptHeader {
 };
 static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
 
+// Checkpoint versions: 1 for FP32, FP16 and BF16 storage; 4 for FX16
+// (power-of-two steps), whose 4 group scales and step count follow the
+// header.
+constexpr int32_t kCkptFx16 = 4;
+static int32_t ckpt_version(int storage) { return storage == TSLB_STORAGE_FX16 ? kCkptFx16 : 1; }
+
 int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -3012,8 +3061,7 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   CkptHeader h;
   memset(&h, 0, sizeof(h));
   memcpy(h.magic, "TSLBCKPT", 8);
-  // version 3: FX16 storage, whose 4 group scales and step count follow the header
-  h.version = c->prm.storage == TSLB_STORAGE_FX16 ? 3 : 1;
+  h.version = ckpt_version(c->prm.storage);
   h.nx = c->prm.nx;
   h.ny = c->prm.ny;
   h.nz = c->prm.nz;
@@ -3027,8 +3075,8 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   const size_t bytes = c->n * c->es;
   std::vector<char> host(bytes);
   bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
-  if (ok && h.version == 3) ok = fwrite(c->kp.sc, sizeof(float), 4, fp) == 4;
-  if (ok && h.version == 3) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
+  if (ok && h.version == kCkptFx16) ok = fwrite(c->kp.sc, sizeof(float), 4, fp) == 4;
+  if (ok && h.version == kCkptFx16) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
   for (int q = 0; ok && q < TSLB_NFIELDS; q++) {
     hipError_t e = hipMemcpyAsync(host.data(), (char *)c->buf[c->cur] + (size_t)q * c->fs * c->es, bytes,
                                   hipMemcpyDeviceToHost, c->stream);
@@ -3052,7 +3100,7 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
   if (!fp) return fail(TSLB_E_ARG, std::string("cannot open ") + path);
   CkptHeader h;
   if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0 ||
-      h.version != (h.storage == TSLB_STORAGE_FX16 ? 3 : 1)) {
+      h.version != ckpt_version(h.storage)) {
     fclose(fp);
     return fail(TSLB_E_ARG, std::string(path) + " is not a TSLB checkpoint");
   }
@@ -3063,18 +3111,19 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
   }
   float sc[4];
   uint32_t kstep = 0;
-  if (h.version == 3 && (fread(sc, sizeof(float), 4, fp) != 4 || fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
+  if (h.version == kCkptFx16 && (fread(sc, sizeof(float), 4, fp) != 4 || fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
     fclose(fp);
     return fail(TSLB_E_ARG, std::string("truncated checkpoint ") + path);
   }
-  if (h.version == 3) {
+  if (h.version == kCkptFx16) {
     for (int g = 0; g < 4; g++) {
-      if (!(sc[g] > 0.f) || !(sc[g] < 1e30f)) {
+      int e;
+      if (!(sc[g] > 0.f) || !(sc[g] < 1e30f) || frexpf(sc[g], &e) != 0.5f) {
         fclose(fp);
         return fail(TSLB_E_ARG, std::string(path) + " holds an invalid FX16 scale");
       }
     }
-    for (int g = 0; g < 4; g++) set_scale(c->kp, TSLB_STORAGE_FX16, g, sc[g]);
+    for (int g = 0; g < 4; g++) set_scale(c->kp, g, sc[g]);
     c->kp.step = kstep;
   }
   const size_t bytes = c->n * c->es;
@@ -3248,7 +3297,7 @@ int tslb_set_storage_range(tslb_ctx *c, int field, float half_range) {
   // the group: decode with the old scale into the second buffer (scratch
   // between steps), encode with the new one
   const Params old = c->kp;
-  set_scale(c->kp, TSLB_STORAGE_FX16, sgrp(field), 1.f / half_range);
+  set_scale(c->kp, sgrp(field), ldexpf(1.f, fx16_shift_for(half_range)));
   float *tmp = (float *)c->buf[1 - c->cur];
   const dim3 g((unsigned)((c->n + 255) / 256));
   for (int q = 0; q < TSLB_NFIELDS; q++) {

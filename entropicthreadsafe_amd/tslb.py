@@ -292,8 +292,9 @@ class Lattice:
         _check(self.lib.tslb_upload_plane(self._h, q, int(k), _fptr(a)), "tslb_upload_plane")
 
     def set_storage_range(self, name, half_range):
-        """FX16 storage: half range of the field's group (rho; u, v, w; the six
-        Pi^neq; phi); values beyond saturate, the group is re-encoded."""
+        """FX16 storage: a half range for the field's group (rho; u, v, w; the
+        six Pi^neq; phi), rounded up to the range 32767 * 2^-m of a
+        power-of-two step; values beyond saturate, the group is re-encoded."""
         q = FIELDS.index(name) if isinstance(name, str) else int(name)
         _check(self.lib.tslb_set_storage_range(self._h, q, float(half_range)), "tslb_set_storage_range")
 

@@ -46,12 +46,13 @@ extern "C" {
 #define TSLB_STORAGE_FP32 0
 #define TSLB_STORAGE_BF16 1
 #define TSLB_STORAGE_FP16 2
-/* 16-bit signed fixed point per field: round(value * 32767 / R), R the half
- * range of the field's group (rho-1; u, v, w; the six Pi^neq; phi), defaults
- * 1/4, 0.375, 1/8 and 1.5 (see tslb_set_storage_range), clipped at +-R
- * (tslb_storage_clipped counts it).
- * The step rounds rho-1 and phi stochastically (floor(x + r), r uniform in
- * [0, 1) from a hash of node and step), the other fields and all uploads to
+/* 16-bit signed fixed point per field: k = round(value * 2^m), |k| <= 32767,
+ * with a power-of-two step 2^-m per field group (rho-1; u, v, w; the six
+ * Pi^neq; phi): defaults 2^-17, 2^-17, 2^-18 and 2^-14, ranges +-1/4, +-1/4,
+ * +-1/8 and +-2 (see tslb_set_storage_range); values beyond are clipped and
+ * counted (tslb_storage_clipped).  The step rounds rho-1, u, v, w and phi
+ * stochastically (to nearest after adding r uniform in (-1/2, 1/2) steps
+ * from a hash of node and step: unbiased), Pi^neq and all uploads to
  * nearest. */
 #define TSLB_STORAGE_FX16 3
 
@@ -126,8 +127,8 @@ int tslb_download_field(tslb_ctx *ctx, int field, float *data);
 /* Moment-space checkpoint, in place of the population dump and reload
  * dump_distros_1c_3d / read_distros_1c_3d (print_module.f90:700-723).  The
  * file holds a 64-byte header (magic "TSLBCKPT", version 1, nx, ny, nz,
- * ncomp, storage, bytes per value, field count, the caller's step; version 3
- * for FX16 storage, whose 4 float group scales 1/R and the library's step
+ * ncomp, storage, bytes per value, field count, the caller's step; version 4
+ * for FX16 storage, whose 4 float group scales 2^m and the library's step
  * count (uint32, the rounding dither) follow the header and are restored on
  * reading) and the 11 stored fields verbatim (storage type,
  * field 0 as rho-1), so a run that
@@ -200,9 +201,10 @@ int tslb_set_jet_bc(tslb_ctx *ctx, int enable);
  * (boundary_cds_3D_module.f90:866-877). */
 int tslb_upload_plane(tslb_ctx *ctx, int field, int k, const float *data);
 
-/* FX16 storage: the half range R of the group of `field` (rho-1; u, v, w;
- * the six Pi^neq; phi), resolution R / 32767, values beyond +-R saturate.
- * The stored fields of the group are re-encoded.  No counterpart in
+/* FX16 storage: a half range R for the group of `field` (rho-1; u, v, w;
+ * the six Pi^neq; phi).  The group takes the finest power-of-two step 2^-m
+ * whose range 32767 2^-m covers R; values beyond saturate.  The stored
+ * fields of the group are re-encoded.  No counterpart in
  * the reference (which stores FP32, vars_single_3D_module.f90:4): the
  * reduced-precision storage of the moment fields. */
 int tslb_set_storage_range(tslb_ctx *ctx, int field, float half_range);
