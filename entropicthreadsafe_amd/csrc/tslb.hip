@@ -139,7 +139,7 @@ struct Params {
   float tau;                // relaxation time (learned closure: omega = 1/(tau + 3 nu_t))
   float sc[4];   // storage scale per field group, sgrp(q) (1 for FP32)
   float isc[4];  // decode scale: 1 / sc
-  uint32_t step;            // fused launches so far (FX16: the dither of the stored rho - 1 and phi)
+  uint32_t step;            // fused launches so far (16-bit storage: the seed of the unbiased rounding)
   unsigned int *sat;        // FX16: count of stored values clipped at their range (device)
 };
 
@@ -195,6 +195,21 @@ __device__ __forceinline__ bf16_t from_f<bf16_t>(float v) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return bf16_t{(uint16_t)(u >> 16)};
 }
+// FP16 and BF16 with stochastic rounding (v_cvt_sr_f16_f32 and
+// v_cvt_sr_bf16_f32): the random bits r decide between the two neighbours
+// in proportion to the distances, so the stored value is unbiased.
+typedef _Float16 f16x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+template <typename S>
+__device__ S from_f_sr(float v, uint32_t r);
+template <>
+__device__ __forceinline__ _Float16 from_f_sr<_Float16>(float v, uint32_t r) {
+  return __builtin_amdgcn_cvt_sr_f16_f32(f16x2_t{}, v, r, false)[0];
+}
+template <>
+__device__ __forceinline__ bf16_t from_f_sr<bf16_t>(float v, uint32_t r) {
+  return bf16_t{__builtin_bit_cast(uint16_t, __builtin_amdgcn_cvt_sr_bf16_f32(bf16x2_t{}, v, r, false)[0])};
+}
 // FX16 encode by the magic number C = 1.5 2^23: for |y| < 2^22 the float
 // y + C lies in [2^23, 2^24), where the spacing is 1, so the one rounding of
 // the sum is round-to-nearest-even of y, and its low 16 bits are k in two's
@@ -208,37 +223,54 @@ template <>
 __device__ __forceinline__ fx16_t from_f<fx16_t>(float v) {
   return fx16_code(__builtin_amdgcn_fmed3f(v, -32767.f, 32767.f));
 }
-// FX16 with unbiased (stochastic) rounding: round-to-nearest of y + e with e
-// uniform in (-1/2, 1/2) takes the upper neighbour of y with probability
-// frac(y), so its mean is y.  Round-to-nearest alone is biased wherever a
-// field is nearly uniform (every node of a resting phase rounds the same
-// way), which drifts the conserved sums, and it drops every per-step change
-// below half a step (a slowly forced flow stagnates); the fused pass stores
-// rho - 1, u, v, w and phi this way.  e = (w + 1/2) 2^-16 - 1/2 for a 16-bit
-// word w of a hash of the node and the step; the caller passes
-// yh = y - 1/2 + 2^-17 (folded into the fma that scales the value).
-constexpr float kFx16DitherShift = -0.5f + 0x1p-17f;
-__device__ __forceinline__ fx16_t fx16_dither(float yh, uint32_t w16) {
-  return fx16_code(fmaf((float)w16, 0x1p-16f, yh));
+// FX16 with unbiased (stochastic) rounding, in integers: Y = y 2^16 rounded
+// to nearest (exact once |y| >= 128), k = floor((Y + w) / 2^16) for a word
+// w uniform in [0, 2^16): k exceeds floor(Y / 2^16) with probability
+// (Y mod 2^16) / 2^16, so the mean of k is Y / 2^16.  Round-to-nearest
+// alone is biased wherever a field is nearly uniform (every node of a
+// resting phase rounds the same way), which drifts the conserved sums, and
+// it drops every per-step change below half a step (a slowly forced flow
+// stagnates); the fused pass stores rho - 1, u, v, w and phi this way.  (The
+// same in floats, round(y - 1/2 + (w + 1/2) 2^-16), loses the dither's low
+// bits in the sum once |y| is large: a bias of ~2.5e-4 steps at phi ~ 1,
+// which drifted the emulsion's phase mass by 1e-4 over 20 000 steps.)
+// |y| <= 32766 keeps Y + w within int32 and k within +-32767.
+__device__ __forceinline__ fx16_t fx16_dither(float y, uint32_t w16) {
+  const int Y = (int)rintf(y * 65536.f);
+  return fx16_t{(int16_t)((Y + (int)w16) >> 16)};
 }
-// The hash words must average to uniform over the lattice step by step:
-// with the seed XORed into the node index, the set of hashed values over a
-// power-of-two lattice is the same every step, and the small deviation of
-// its mean from uniform (~1e-5) repeated every step (a phase-mass drift of
-// 1e-3 over 20 000 steps of the 512^3 emulsion).  Adding the seed moves the
-// window of hashed values instead, so the deviation changes sign and size
-// from step to step and averages out.
-__device__ __forceinline__ uint32_t fx16_hash(uint32_t node, uint32_t seed) {
-  const uint32_t h = (node + seed) * 0x9E3779B1u;
-  return h ^ (h >> 16);
+// The random words of the unbiased rounding: a hash of node and step.
+// Two properties matter.  (1) The words must average to uniform over the
+// lattice step by step: with the seed XORed into the node index, the set of
+// hashed values over a power-of-two lattice is the same every step, and the
+// small deviation of its mean from uniform (~1e-5) repeated every step (a
+// phase-mass drift of 1e-3 over 20 000 steps of the 512^3 emulsion); adding
+// the seed moves the window of hashed values instead.  (2) A node's words
+// must be unrelated from one step to the next: with one multiplication
+// (node + step K) times a constant, the high half moved by a constant each
+// step, and the rounding of u, fed back through the flow, drifted a uniform
+// flow by 0.03 steps per step.  mix32 is the two-round multiply-xorshift
+// finaliser "lowbias32" (C. Wellons, hash-prospector), applied to the step
+// (a scalar) and to node + seed.
+__host__ __device__ constexpr uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
 }
-__device__ __forceinline__ uint32_t fx16_rehash(uint32_t h) {
-  h *= 0x85EBCA6Bu;
-  return h ^ (h >> 13);
+__device__ __forceinline__ uint32_t node_hash(uint32_t node, uint32_t step) { return mix32(node + mix32(step)); }
+// 32 more bits from a mixed word
+__device__ __forceinline__ uint32_t node_rehash(uint32_t h) {
+  h = (h ^ (h >> 15)) * 0x2C1B3C6Du;
+  return h ^ (h >> 12);
 }
-// the fields stored with unbiased rounding and the hash word each takes
-__host__ __device__ constexpr bool fx16_dithered(int q) { return q < 4 || q == TSLB_PHI; }
-__host__ __device__ constexpr int fx16_word(int q) { return q < 4 ? q : 4; }
+// the fields that the fused pass stores with unbiased rounding (every
+// 16-bit format) and the hash word each takes
+__host__ __device__ constexpr bool dithered(int q) { return q < 4 || q == TSLB_PHI; }
+__host__ __device__ constexpr int dither_word(int q) { return q < 4 ? q : 4; }
+__device__ __forceinline__ uint32_t rot16(uint32_t h) { return (h << 16) | (h >> 16); }
 
 // a stored value of field group g, decoded
 template <typename S>
@@ -1430,42 +1462,46 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
                                aA.syz - rho * v * w,
                                NC == 3 ? fmaf(2.f * aA.phi, inv, -1.f) : aA.phi};
         if constexpr (std::is_same<S, fx16_t>::value) {
-          // the value in steps, y = v 2^m (yh = y - 1/2 + 2^-17 where it is
-          // dithered)
+          // the value in steps, y = v 2^m
           float x[NF];
 #pragma unroll
-          for (int q = 0; q < NF; q++)
-            x[q] = fx16_dithered(q) ? fmaf(out[q], p.sc[sgrp(q)], kFx16DitherShift) : out[q] * p.sc[sgrp(q)];
-          // Past |x| = 32766 the values are clamped so that neither rounding
-          // leaves +-32767 (a sum rounded to 32767.5 would round to the even
-          // 32768); a node with a value beyond its field's range, |y| >
-          // 32767.5, is counted.
+          for (int q = 0; q < NF; q++) x[q] = out[q] * p.sc[sgrp(q)];
+          // Past |y| = 32766 the values are clamped so that neither rounding
+          // leaves +-32767 (dithered ones to 32766, see fx16_dither); a node
+          // with a value beyond its field's range, |y| > 32767.5, is counted.
           float mx = 0.f;
 #pragma unroll
           for (int q = 0; q < NF; q++) mx = fmaxf(mx, fabsf(x[q]));
           if (mx > 32766.f) {
             bool clip = false;
 #pragma unroll
-            for (int q = 0; q < NF; q++)
-              clip |= fabsf(fx16_dithered(q) ? x[q] - kFx16DitherShift : x[q]) > 32767.5f;
+            for (int q = 0; q < NF; q++) clip |= fabsf(x[q]) > 32767.5f;
             if (clip && p.sat) atomicAdd(p.sat, 1u);
 #pragma unroll
             for (int q = 0; q < NF; q++)
-              x[q] = __builtin_amdgcn_fmed3f(x[q], -32767.f, fx16_dithered(q) ? 32766.f : 32767.f);
+              x[q] = __builtin_amdgcn_fmed3f(x[q], -32767.f, dithered(q) ? 32766.f : 32767.f);
           }
-          const uint32_t h1 = fx16_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step * 0x632BE5ABu);
-          const uint32_t h2 = fx16_rehash(h1);
+          const uint32_t h1 = node_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step);
+          const uint32_t h2 = node_rehash(h1);
           const uint32_t w[5] = {h1 & 0xFFFFu, h1 >> 16, h2 & 0xFFFFu, h2 >> 16, (h1 ^ h2) & 0xFFFFu};
 #pragma unroll
           for (int q = 0; q < NF; q++)
             bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS,
-                   fx16_dithered(q) ? fx16_dither(x[q], w[fx16_word(q)]) : fx16_code(x[q]));
+                   dithered(q) ? fx16_dither(x[q], w[dither_word(q)]) : fx16_code(x[q]));
+        } else if constexpr (sizeof(S) == 4) {
+#pragma unroll
+          for (int q = 0; q < NF; q++) bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS, from_f<S>(out[q]));
         } else {
+          // FP16, BF16: rho - 1, u, v, w and phi with stochastic rounding,
+          // 32 random bits each
+          const uint32_t h1 = node_hash(d_xy + (uint32_t)plane * (uint32_t)zd, p.step);
+          const uint32_t h2 = node_rehash(h1);
+          const uint32_t rw[5] = {h1, h2, rot16(h1), rot16(h2), h1 ^ h2};
 #pragma unroll
           for (int q = 0; q < NF; q++) {
-            const rsrc_t r = rsrc((char *)dst + zb + q * fsS, pbS);
-            if constexpr (sizeof(S) == 4) bst<2>(r, offS, from_f<S>(out[q]));
-            else bst<2>(r, offS, from_f<S>(out[q] * p.sc[sgrp(q)]));
+            const float v = out[q] * p.sc[sgrp(q)];
+            bst<2>(rsrc((char *)dst + zb + q * fsS, pbS), offS,
+                   dithered(q) ? from_f_sr<S>(v, rw[dither_word(q)]) : from_f<S>(v));
           }
         }
         if constexpr (NC == 2) hit = out[10] * (1.f - out[10]) > 0.19f;
@@ -3048,11 +3084,11 @@ struct CkptHeader {
 };
 static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
 
-// Checkpoint versions: 1 for FP32, FP16 and BF16 storage; 4 for FX16
-// (power-of-two steps), whose 4 group scales and step count follow the
-// header.
-constexpr int32_t kCkptFx16 = 4;
-static int32_t ckpt_version(int storage) { return storage == TSLB_STORAGE_FX16 ? kCkptFx16 : 1; }
+// Checkpoint versions: 1 for FP32 storage; 4 for the 16-bit formats, whose
+// 4 group scales (FX16: power-of-two steps) and step count (the seed of the
+// stochastic rounding) follow the header.
+constexpr int32_t kCkpt16 = 4;
+static int32_t ckpt_version(int storage) { return storage == TSLB_STORAGE_FP32 ? 1 : kCkpt16; }
 
 int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   int rc = check_ctx(c);
@@ -3075,8 +3111,8 @@ int tslb_checkpoint_write(tslb_ctx *c, const char *path, int64_t step) {
   const size_t bytes = c->n * c->es;
   std::vector<char> host(bytes);
   bool ok = fwrite(&h, sizeof(h), 1, fp) == 1;
-  if (ok && h.version == kCkptFx16) ok = fwrite(c->kp.sc, sizeof(float), 4, fp) == 4;
-  if (ok && h.version == kCkptFx16) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
+  if (ok && h.version == kCkpt16) ok = fwrite(c->kp.sc, sizeof(float), 4, fp) == 4;
+  if (ok && h.version == kCkpt16) ok = fwrite(&c->kp.step, sizeof(uint32_t), 1, fp) == 1;
   for (int q = 0; ok && q < TSLB_NFIELDS; q++) {
     hipError_t e = hipMemcpyAsync(host.data(), (char *)c->buf[c->cur] + (size_t)q * c->fs * c->es, bytes,
                                   hipMemcpyDeviceToHost, c->stream);
@@ -3111,16 +3147,16 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
   }
   float sc[4];
   uint32_t kstep = 0;
-  if (h.version == kCkptFx16 && (fread(sc, sizeof(float), 4, fp) != 4 || fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
+  if (h.version == kCkpt16 && (fread(sc, sizeof(float), 4, fp) != 4 || fread(&kstep, sizeof(uint32_t), 1, fp) != 1)) {
     fclose(fp);
     return fail(TSLB_E_ARG, std::string("truncated checkpoint ") + path);
   }
-  if (h.version == kCkptFx16) {
+  if (h.version == kCkpt16) {
     for (int g = 0; g < 4; g++) {
       int e;
       if (!(sc[g] > 0.f) || !(sc[g] < 1e30f) || frexpf(sc[g], &e) != 0.5f) {
         fclose(fp);
-        return fail(TSLB_E_ARG, std::string(path) + " holds an invalid FX16 scale");
+        return fail(TSLB_E_ARG, std::string(path) + " holds an invalid storage scale");
       }
     }
     for (int g = 0; g < 4; g++) set_scale(c->kp, g, sc[g]);

@@ -44,6 +44,10 @@ extern "C" {
 #define TSLB_COLOUR 3
 
 #define TSLB_STORAGE_FP32 0
+/* BF16 and FP16 (rho-1 x64 and Pi^neq x4096 to stay in FP16's normal
+ * range): the step stores rho-1, u, v, w and phi with stochastic rounding
+ * (unbiased, random bits from a hash of node and step), the rest and all
+ * uploads to nearest even. */
 #define TSLB_STORAGE_BF16 1
 #define TSLB_STORAGE_FP16 2
 /* 16-bit signed fixed point per field: k = round(value * 2^m), |k| <= 32767,
@@ -128,9 +132,9 @@ int tslb_download_field(tslb_ctx *ctx, int field, float *data);
  * dump_distros_1c_3d / read_distros_1c_3d (print_module.f90:700-723).  The
  * file holds a 64-byte header (magic "TSLBCKPT", version 1, nx, ny, nz,
  * ncomp, storage, bytes per value, field count, the caller's step; version 4
- * for FX16 storage, whose 4 float group scales 2^m and the library's step
- * count (uint32, the rounding dither) follow the header and are restored on
- * reading) and the 11 stored fields verbatim (storage type,
+ * for the 16-bit formats, whose 4 float group scales (FX16: 2^m) and the
+ * library's step count (uint32, the seed of the stochastic rounding) follow
+ * the header and are restored on reading) and the 11 stored fields verbatim (storage type,
  * field 0 as rho-1), so a run that
  * reads it continues bit for bit as one that never stopped.  The isfluid map
  * and plane BCs are not part of it: the caller sets them again.  Reading

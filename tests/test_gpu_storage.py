@@ -26,11 +26,11 @@ pytestmark = pytest.mark.gpu
 
 # (rho, u, Pi) bounds per storage: 1.5x the larger of the errors measured on
 # the MI355X in the Taylor-Green and channel cases below
-TOL = {"fp16": (1.8e-4, 2.4e-3, 2.1e-2), "bf16": (7.9e-4, 4.3e-2, 0.18), "fx16": (6.1e-5, 1.3e-3, 7.6e-3)}
+TOL = {"fp16": (1.6e-4, 3.7e-3, 2.6e-2), "bf16": (1.3e-3, 2.7e-2, 0.2), "fx16": (5.3e-5, 1.3e-3, 8.3e-3)}
 # the same at the C3 size, where Pi^neq carries the noise of the stored
 # velocity's rounding (test_c3_channel_storage_error); FX16 meets the 1e-3
-# target for rho and u (measured 5.2e-5 and 4.6e-4)
-TOL_C3 = {"fp16": (2.2e-4, 3.2e-3, 0.2), "bf16": (2.0e-3, 3.6e-2, 2.0), "fx16": (7.8e-5, 7.0e-4, 8.7e-2)}
+# target for rho and u (measured 5.9e-5 and 4.5e-4)
+TOL_C3 = {"fp16": (3.0e-4, 4.5e-3, 0.33), "bf16": (2.7e-3, 3.4e-2, 2.4), "fx16": (8.8e-5, 6.8e-4, 9.1e-2)}
 STORAGES = ["fp16", "bf16", "fx16"]
 
 
@@ -100,6 +100,34 @@ def test_channel_reduced_storage_vs_fp64(storage):
 
 
 @pytest.mark.parametrize("storage", STORAGES)
+def test_slow_forcing_moves_a_uniform_flow(storage):
+    """A uniform flow u0 = 0.05 under a body force of 2e-7 per step, fully
+    periodic: over 2000 steps its velocity grows by 4e-4, every step far
+    below half a storage step at u0 (BF16 2.4e-4, FP16 3.1e-5, FX16
+    7.6e-6).  Rounded to nearest, the stored velocity never moves; the
+    fused pass rounds u stochastically, so its mean grows as in FP32
+    storage, within the noise of the rounding (~5 % for BF16 over these
+    65 536 nodes)."""
+    nx, ny, nz = 64, 32, 32
+    fx, steps, u0 = 2e-7, 2000, 0.05
+    shape = (nz, ny, nx)
+    du = {}
+    for st in ("fp32", storage):
+        with Lattice(nx, ny, nz, tau=0.8, force=(fx, 0.0, 0.0), storage=st) as lat:
+            lat.upload_field("rho", np.ones(shape, np.float32))
+            lat.upload_field("u", np.full(shape, u0, np.float32))
+            lat.upload_field("v", np.zeros(shape, np.float32))
+            lat.upload_field("w", np.zeros(shape, np.float32))
+            lat.init_equilibrium(False)
+            a = lat.download_field("u").astype(np.float64).mean()
+            lat.step(steps)
+            du[st] = lat.download_field("u").astype(np.float64).mean() - a
+    print(storage, "mean u growth", du[storage], "fp32", du["fp32"], "force x steps", steps * fx)
+    assert abs(du["fp32"] / (steps * fx) - 1) < 0.01
+    assert abs(du[storage] / du["fp32"] - 1) < 0.25
+
+
+@pytest.mark.parametrize("storage", STORAGES)
 def test_droplet_reduced_storage_conserves_mass(storage):
     case = configs.droplet(n=32, radius=8.0, sigma=0.02, beta=0.7, noise=1e-3)
     with Lattice(32, 32, 32, storage=storage, **case.lattice) as lat:
@@ -128,8 +156,8 @@ def test_c3_channel_storage_error(storage):
     Here Pi^neq (~nu du/dy ~ 4e-6) is of the order of the velocity gradients
     that the rounding of the stored u makes (FP16 steps 6e-5 at u ~ 0.08,
     BF16 5e-4), so Pi^neq carries that noise: the bounds on Pi are far above
-    the small cases' (measured FP16: rho 1.5e-4, u 2.0e-3, Pi 0.16 max; RMS
-    2.3e-5, 7.8e-4, 0.03-0.12)."""
+    the small cases' (measured FP16: rho 2.0e-4, u 3.0e-3, Pi 0.22 max; RMS
+    3.2e-5, 1.1e-3, 0.04-0.17)."""
     case = configs.channel()
     nz, ny, nx = case.shape
     out = {}
