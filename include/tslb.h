@@ -55,9 +55,9 @@ extern "C" {
  * Pi^neq; phi): defaults 2^-17, 2^-17, 2^-18 and 2^-14, ranges +-1/4, +-1/4,
  * +-1/8 and +-2 (see tslb_set_storage_range); values beyond are clipped and
  * counted (tslb_storage_clipped).  The step rounds rho-1, u, v, w and phi
- * stochastically (to nearest after adding r uniform in (-1/2, 1/2) steps
- * from a hash of node and step: unbiased), Pi^neq and all uploads to
- * nearest. */
+ * stochastically (k = floor(value * 2^m + r), r uniform in [0, 1) in steps
+ * of 2^-16 from a hash of node and step: unbiased), Pi^neq and all uploads
+ * to nearest. */
 #define TSLB_STORAGE_FX16 3
 
 /* Moment field indices (the order of tslb_upload_moments' packed buffer). */
