@@ -144,12 +144,14 @@ struct Params {
 };
 
 
-// Moment storage: FP32, FP16, BF16 (round-to-nearest-even) or FX16.  Reduced
-// storage keeps every field pre-multiplied by a per-field scale (Params::sc)
-// so that the shifted density and Pi^neq sit in FP16's normal range; compute
-// is always FP32.  FX16 stores each field as a 16-bit signed fixed-point
-// number k = round(v 2^m), |k| <= 32767: uniform absolute resolution 2^-m (m per field group) instead of 11 (FP16)
-// or 8 (BF16) significant bits.  The scales sc = 2^m and isc = 2^-m are
+// Moment storage: FP32, FP16, BF16 or FX16 (the fused pass rounds rho - 1,
+// u, v, w and phi of the 16-bit formats stochastically, everything else
+// rounds to nearest even).  Reduced storage keeps every field pre-multiplied
+// by a per-group scale (Params::sc) so that the shifted density and Pi^neq
+// sit in FP16's normal range; compute is always FP32.  FX16 stores each
+// field as a 16-bit signed fixed-point number k = round(v 2^m), |k| <=
+// 32767: uniform absolute resolution 2^-m (m per field group) instead of 11
+// (FP16) or 8 (BF16) significant bits.  The scales sc = 2^m and isc = 2^-m are
 // powers of two, so decode and encode are exact inverses: a value that the
 // step leaves unchanged is stored again as the same k, also under the
 // stochastic rounding below.  (With a scale such as 32767 / R the product
