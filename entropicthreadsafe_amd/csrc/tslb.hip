@@ -3137,9 +3137,20 @@ int tslb_checkpoint_read(tslb_ctx *c, const char *path, int64_t *step) {
   FILE *fp = fopen(path, "rb");
   if (!fp) return fail(TSLB_E_ARG, std::string("cannot open ") + path);
   CkptHeader h;
-  if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0 ||
-      h.version != ckpt_version(h.storage)) {
+  if (fread(&h, sizeof(h), 1, fp) != 1 || memcmp(h.magic, "TSLBCKPT", 8) != 0) {
     fclose(fp);
+    return fail(TSLB_E_ARG, std::string(path) + " is not a TSLB checkpoint");
+  }
+  // version 1 files of FP16 and BF16 storage (written before the step count
+  // was stored) hold the same fixed scales: they are read, and the run goes
+  // on with the context's own step count
+  const bool legacy16 =
+      h.version == 1 && (h.storage == TSLB_STORAGE_FP16 || h.storage == TSLB_STORAGE_BF16);
+  if (h.version != ckpt_version(h.storage) && !legacy16) {
+    fclose(fp);
+    if (h.storage == TSLB_STORAGE_FX16 && h.version == 3)
+      return fail(TSLB_E_ARG, std::string(path) + " holds FX16 codes of the earlier range-scaled format (version 3), "
+                                                  "which power-of-two steps cannot decode exactly");
     return fail(TSLB_E_ARG, std::string(path) + " is not a TSLB checkpoint");
   }
   if (h.nx != c->prm.nx || h.ny != c->prm.ny || h.nz != c->prm.nz || h.ncomp != c->prm.ncomp ||

@@ -134,7 +134,8 @@ int tslb_download_field(tslb_ctx *ctx, int field, float *data);
  * ncomp, storage, bytes per value, field count, the caller's step; version 4
  * for the 16-bit formats, whose 4 float group scales (FX16: 2^m) and the
  * library's step count (uint32, the seed of the stochastic rounding) follow
- * the header and are restored on reading) and the 11 stored fields verbatim (storage type,
+ * the header and are restored on reading; version 1 files of FP16 and BF16
+ * storage are read too and continue with the context's step count) and the 11 stored fields verbatim (storage type,
  * field 0 as rho-1), so a run that
  * reads it continues bit for bit as one that never stopped.  The isfluid map
  * and plane BCs are not part of it: the caller sets them again.  Reading
