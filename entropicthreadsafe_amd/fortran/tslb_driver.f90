@@ -27,9 +27,11 @@ program tslb_driver
   ! checkpoint (dump/out_m_.ckpt) every dumpstep steps; restart from a file
   integer :: dumpstep = 0, stamp_vtk = 0
   character(len=256) :: restart = ''
+  ! moment storage: 'fp32' (the reference's REAL(4)), 'fp16', 'bf16' or 'fx16'
+  character(len=8) :: storage = 'fp32'
   namelist /tslb_run/ nx, ny, nz, ncomp, nsteps, stamp2D, periodic, force_mode, tau, sigma, &
     sharp_c, fx, fy, fz, lin_force, config, radius, u0, uwall, noise, dir_out, dumpstep, &
-    stamp_vtk, restart
+    stamp_vtk, restart, storage
 
   type(tslb_params) :: prm
   type(c_ptr) :: ctx
@@ -71,6 +73,19 @@ program tslb_driver
   prm%sharp_c = sharp_c
   prm%force = (/fx, fy, fz/)
   prm%lin_force = lin_force
+  select case (trim(storage))
+  case ('fp32')
+    prm%storage = TSLB_STORAGE_FP32
+  case ('fp16')
+    prm%storage = TSLB_STORAGE_FP16
+  case ('bf16')
+    prm%storage = TSLB_STORAGE_BF16
+  case ('fx16')
+    prm%storage = TSLB_STORAGE_FX16
+  case default
+    write(0, '(a,a)') 'tslb_driver: unknown storage ', trim(storage)
+    error stop 1
+  end select
   allocate(rho(nx,ny,nz), u(nx,ny,nz), v(nx,ny,nz), w(nx,ny,nz), phi(nx,ny,nz))
   allocate(isfluid(nx,ny,nz), w_in(nx,ny), phi_in(nx,ny))
 
@@ -84,6 +99,7 @@ program tslb_driver
   write(6,*) 'config ', trim(config)
   write(6,*) 'nx ny nz', nx, ny, nz
   write(6,*) 'ncomp', ncomp
+  write(6,*) 'storage ', trim(storage)
   write(6,*) 'periodic', periodic
   write(6,*) 'nsteps', nsteps
   write(6,*) 'stamp2D', stamp2D
