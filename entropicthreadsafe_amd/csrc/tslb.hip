@@ -507,9 +507,15 @@ __device__ __forceinline__ void all_pops_mt(const float *m, float Vx, float Vy, 
 // the M[a][b][c] below carry Q, P, z exponents.  rq[q + 1] addresses the LDS
 // slot of the c_z = -1 population moving q along Q; c_z = 0 and +1 sit 8 and
 // 16 slots further (qslot).
-template <bool ADD>
+// Adj: adj(q, cz, v) returns the stored population of the direction with
+// c_Q = q, c_z = cz (the colour model adds its perturbation there).
+struct NoAdj {
+  __device__ __forceinline__ float operator()(int, int, float v) const { return v; }
+};
+template <bool ADD, typename Adj = NoAdj>
 __device__ __forceinline__ void ring_pops(const float *m, float Vx, float Vy, float Vz, const Params &p,
-                                          float om1, bool col, float sgn, float *const (&rq)[3]) {
+                                          float om1, bool col, float sgn, float *const (&rq)[3],
+                                          const Adj &adj = Adj{}) {
   const float uq = col ? m[2] : m[1], up = col ? m[1] : m[2];
   const float Pqq = col ? m[5] : m[4], Ppp = col ? m[4] : m[5];
   const float Pqz = col ? m[9] : m[8], Ppz = col ? m[8] : m[9];
@@ -537,11 +543,41 @@ __device__ __forceinline__ void ring_pops(const float *m, float Vx, float Vy, fl
 #pragma unroll
   for (int cz = 0; cz < 3; cz++) {
     const float m0 = G[0][cz], m1 = G[1][cz], m2 = G[2][cz];
-    rq[0][cz * 8 * NSLOT] = m2 - m1;
-    rq[1][cz * 8 * NSLOT] = fmaf(-2.f, m2, m0);
-    rq[2][cz * 8 * NSLOT] = m2 + m1;
+    rq[0][cz * 8 * NSLOT] = adj(-1, cz - 1, m2 - m1);
+    rq[1][cz * 8 * NSLOT] = adj(0, cz - 1, fmaf(-2.f, m2, m0));
+    rq[2][cz * 8 * NSLOT] = adj(1, cz - 1, m2 + m1);
   }
 }
+
+// Colour model, ring sources: the entering direction with c_Q = q, c_P =
+// sgn, c_z = cz, its weight, 1/|c| and c.g (the ring side's P axis is y for
+// rows, x for columns).
+struct RingDir {
+  bool col;
+  float sgn, gx, gy, gz;
+  __device__ __forceinline__ float cg(int q, int cz) const {
+    const float cx = col ? sgn : (float)q, cy = col ? (float)q : sgn;
+    return fmaf(cx, gx, fmaf(cy, gy, (float)cz * gz));
+  }
+  // c_P != 0: one, two or three nonzero components
+  __device__ __forceinline__ static float w(int q, int cz) {
+    const int nnz = 1 + (q != 0) + (cz != 0);
+    return nnz == 1 ? 2.f / 27.f : (nnz == 2 ? 1.f / 54.f : 1.f / 216.f);
+  }
+  __device__ __forceinline__ static float inv(int q, int cz) {
+    const int nnz = 1 + (q != 0) + (cz != 0);
+    return nnz == 1 ? 1.f : (nnz == 2 ? 0.70710678118654752f : 0.57735026918962576f);
+  }
+};
+// the colour-gradient perturbation of ColourSink on one entering population
+struct ColourRingAdj {
+  RingDir d;
+  float K, g2;
+  __device__ __forceinline__ float operator()(int q, int cz, float v) const {
+    const float cg = d.cg(q, cz);
+    return fmaf(K, RingDir::w(q, cz) * fmaf(cg, cg, -g2), v);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Colour-gradient stencil, grouped as the reference (recursive_2c_TSLB3D.f90:
@@ -910,11 +946,30 @@ __device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p,
 template <int NC, bool ADD, bool CLO>
 __device__ __forceinline__ void eval_ring(const Src<NC, CLO> &s, const Params &p, bool col, float sgn,
                                           float *const (&rq)[3], float *rf) {
-  static_assert(NC == 1 || NC == 2, "the colour model's ring keeps the full evaluation");
   const float *m = s.m;
   float cbuf[10], om1;
   const float *mc = closure_moments(s, p, cbuf, om1);
-  if constexpr (NC == 2) {
+  if constexpr (NC == 3) {
+    // the colour model (eval_src's, on the 9 entering populations); rf
+    // returns cA, bc, gx, gy, gz for the recolouring pass
+    float ca = 0.5f * (1.f + m[10]);
+    ca = fminf(fmaxf(ca, 0.f), 1.f);
+    const float cb = 1.f - ca;
+    const float nu = 1.f / (ca / p.nua + cb / p.nub);
+    const float omega = 1.f / fmaf(3.f, nu, 0.5f);
+    const float gx = s.nn[0], gy = s.nn[1], gz = s.nn[2];
+    const float g2 = gx * gx + gy * gy + gz * gz;
+    const float mg = sqrtf(g2);
+    const bool pert = mg > 0.001f;
+    const float K = pert ? 2.25f * p.sigma * omega / mg : 0.f;
+    const ColourRingAdj adj{RingDir{col, sgn, gx, gy, gz}, K, g2};
+    ring_pops<ADD>(m, m[1], m[2], m[3], p, 1.f - omega, col, sgn, rq, adj);
+    rf[0] = ca;
+    rf[1] = pert ? p.beta * ca * cb * (1.f + m[0]) / mg : 0.f;
+    rf[2] = gx;
+    rf[3] = gy;
+    rf[4] = gz;
+  } else if constexpr (NC == 2) {
     const float phi = m[10];
     const float n0 = s.nn[0], n1 = s.nn[1], n2 = s.nn[2];
     const float curv = s.kap;
@@ -1030,6 +1085,20 @@ __device__ __forceinline__ void colour_rewrite(float (*lds)[NSLOT], int slot, co
       lds[qslot(L)][slot] = colour_a<L>(lds[qslot(L)][slot], c[0], c[1], c[2], c[3], c[4]);
     colour_rewrite<L + 1>(lds, slot, c);
   }
+}
+
+// colour model, a ring source's second pass: its 9 entering total
+// populations become recoloured A populations (colour_a by direction)
+__device__ __forceinline__ void colour_rewrite_ring(float *const (&rq)[3], const float *c, bool col, float sgn) {
+  const RingDir d{col, sgn, c[2], c[3], c[4]};
+#pragma unroll
+  for (int cz = -1; cz <= 1; cz++)
+#pragma unroll
+    for (int q = -1; q <= 1; q++) {
+      float &v = rq[q + 1][(cz + 1) * 8 * NSLOT];
+      const float w = RingDir::w(q, cz);
+      v = fmaf(c[0], v + w, c[1] * (w * RingDir::inv(q, cz)) * d.cg(q, cz));
+    }
 }
 
 // colour model: A populations of the c_z = CZ layer arriving at (tx, ty)
@@ -1154,6 +1223,18 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
     else if (tid < 2 * SW + TY) { rsx = 0; rsy = 1 + (tid - 2 * SW); }
     else { rsx = SW - 1; rsy = 1 + (tid - 2 * SW - TY); }
   }
+  // a ring source's side (columns: c_P along x; rows: along y), the sign of
+  // the c_P that enters the tile, and the LDS addresses of its 9 entering
+  // populations (c_Q = -1, 0, 1 at c_z = -1; c_z = 0 and 1 sit 8 and 16
+  // slots further)
+  auto ring_geom = [&](bool &col, float &sgn, float *(&rq)[3]) {
+    col = tid >= 2 * SW;
+    sgn = (tid < SW || (tid >= 2 * SW && tid < 2 * SW + TY)) ? 1.f : -1.f;
+    const int sg = sgn > 0.f ? 1 : -1, rs = rsy * SW + rsx;
+    rq[0] = &lds[col ? dslot(sg, -1) : dslot(-1, sg)][rs];
+    rq[1] = &lds[col ? dslot(sg, 0) : dslot(0, sg)][rs];
+    rq[2] = &lds[col ? dslot(sg, 1) : dslot(1, sg)][rs];
+  };
   int rx = 0, ry = 0;
   const bool rxy = tid < NRING && src_coord(x0 + rsx - 1, p.nx, p.px, rx) &&
                    src_coord(y0 + rsy - 1, p.ny, p.py, ry);
@@ -1333,14 +1414,15 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         cr.nn[2] = nnr[2];
         cr.m[NF - 1] = p0[cr_c];
       }
-      if constexpr (R9 && NC <= 2) {
-        // the 9 populations (and the phase flux) that enter the tile
-        const bool col = tid >= 2 * SW;
-        const float sgn = (tid < SW || (tid >= 2 * SW && tid < 2 * SW + TY)) ? 1.f : -1.f;
-        const int sg = sgn > 0.f ? 1 : -1, rs = rsy * SW + rsx;
-        float *const rq[3] = {&lds[col ? dslot(sg, -1) : dslot(-1, sg)][rs], &lds[col ? dslot(sg, 0) : dslot(0, sg)][rs],
-                              &lds[col ? dslot(sg, 1) : dslot(1, sg)][rs]};
-        eval_ring<NC, ADD>(cr, p, col, sgn, rq, &lds[col ? 24 : 25][rs]);
+      if constexpr (R9) {
+        // the 9 populations (and the phase flux, or the colour model's
+        // recolouring inputs) that enter the tile
+        bool col;
+        float sgn;
+        float *rq[3];
+        ring_geom(col, sgn, rq);
+        if constexpr (NC == 3) eval_ring<NC, ADD>(cr, p, col, sgn, rq, rc);
+        else eval_ring<NC, ADD>(cr, p, col, sgn, rq, &lds[col ? 24 : 25][rsy * SW + rsx]);
       } else {
         PopSink sk{lds, rsy * SW + rsx, 0.f, 0.f, 0.f};
         eval_src<NC, ADD>(cr, p, sk, gq);
@@ -1406,7 +1488,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
         a5 = colour_a<5>(h5, oc[0], oc[1], oc[2], oc[3], oc[4]);
         a6 = colour_a<6>(h6, oc[0], oc[1], oc[2], oc[3], oc[4]);
       }
-      if (cr_ok) colour_rewrite<1>(lds, rsy * SW + rsx, rc);
+      if (cr_ok) {
+        if constexpr (R9) {
+          bool col;
+          float sgn;
+          float *rq[3];
+          ring_geom(col, sgn, rq);
+          colour_rewrite_ring(rq, rc, col, sgn);
+        } else {
+          colour_rewrite<1>(lds, rsy * SW + rsx, rc);
+        }
+      }
       __builtin_amdgcn_s_setprio(0);
       __syncthreads();
       if (active) {

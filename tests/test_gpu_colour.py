@@ -155,8 +155,10 @@ def test_colour_sphere_at_config_2_size():
 # 1.5x the MI355X measurements:
 #   max (rho, u, Pi, psi)   FP16 3.2e-5 5.2e-3 2.0e-3 7.5e-4, BF16 2.0e-4
 #   4.2e-2 9.0e-3 7.2e-3, FX16 2.7e-5 7.9e-3 1.7e-3 1.0e-4;
-#   mass / A mass drift     FP16 6.0e-8 / 2.0e-4, BF16 1.1e-6 / 1.2e-3,
-#   FX16 2.3e-7 / 2.9e-5.
+#   mass / A mass drift     FP16 1.3e-7 / 1.6e-4, BF16 1.1e-6 / 7.2e-4,
+#   FX16 1.1e-7 / 1.7e-5; these are random walks of the stochastic
+#   rounding (one build measured FP16 6.0e-8 / 2.0e-4, BF16 1.1e-6 / 1.2e-3,
+#   FX16 2.3e-7 / 2.9e-5), so their bounds are 3x the larger measurement.
 # psi = -1 + 2 rho_A / rho near -1 in the B phase: rounded to nearest, FP16
 # steps of 4.9e-4 (BF16 3.9e-3) round the A traces there away each step (A
 # mass -2.2e-2 and -4.6e-2 over the 500 steps); stochastic rounding keeps
@@ -166,12 +168,12 @@ def test_colour_sphere_at_config_2_size():
 # 7.6e-6), where FP16's relative resolution is finer;
 # tslb_set_storage_range narrows it.
 STORAGE_TOL = {
-    "fp16": dict(max=(4.8e-5, 7.8e-3, 3.0e-3, 1.1e-3), rms=(4.5e-6, 3.2e-3, 2.0e-3, 3.0e-4), mass=9.1e-8,
-                 amass=3.1e-4),
-    "bf16": dict(max=(3.0e-4, 6.3e-2, 1.3e-2, 1.1e-2), rms=(2.6e-5, 2.3e-2, 1.3e-2, 2.1e-3), mass=1.6e-6,
-                 amass=1.8e-3),
-    "fx16": dict(max=(4.1e-5, 1.2e-2, 2.6e-3, 1.5e-4), rms=(6.7e-6, 8.6e-3, 3.0e-3, 4.3e-5), mass=3.4e-7,
-                 amass=4.3e-5),
+    "fp16": dict(max=(4.8e-5, 7.8e-3, 3.0e-3, 1.1e-3), rms=(4.5e-6, 3.2e-3, 2.0e-3, 3.0e-4), mass=4e-7,
+                 amass=6e-4),
+    "bf16": dict(max=(3.0e-4, 6.3e-2, 1.3e-2, 1.1e-2), rms=(2.6e-5, 2.3e-2, 1.3e-2, 2.1e-3), mass=3.4e-6,
+                 amass=3.6e-3),
+    "fx16": dict(max=(4.1e-5, 1.2e-2, 2.6e-3, 1.5e-4), rms=(6.7e-6, 8.6e-3, 3.0e-3, 4.3e-5), mass=7e-7,
+                 amass=9e-5),
 }
 GROUPS = (("rho",), ("u", "v", "w"), ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz"), ("phi",))
 
