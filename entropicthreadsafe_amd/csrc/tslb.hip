@@ -852,14 +852,56 @@ __device__ __forceinline__ const float *closure_moments(const Src<NC, CLO> &s, c
 template <typename Inner>
 struct ColourSink {
   Inner &in;
-  float K, gx, gy, gz, g2;
+  // K w_l ((c_l.g)^2 - |g|^2) needs only 13 distinct (c.g)^2 - |g|^2, all
+  // from the squares and pair products of g: faces -(g_b^2 + g_c^2), edges
+  // -g_c^2 +- 2 g_a g_b, corners 2 (sx sy gx gy + sx sz gx gz + sy sz gy gz)
+  // (4 sign patterns up to a common sign), the rest (10/27) |g|^2
+  float kf, ke, kc, k0;  // K w by class, and K (10/27) |g|^2
+  float fa[3];           // faces x, y, z
+  float ep[3], em[3];    // edges xy, xz, yz with sa sb = +1, -1
+  float cn[4];           // corners (+,+,+), (+,+,-), (+,-,+), (-,+,+)
+  __device__ __forceinline__ ColourSink(Inner &i, float K, float gx, float gy, float gz, float g2) : in(i) {
+    const float x2 = gx * gx, y2 = gy * gy, z2 = gz * gz;
+    const float pxy = 2.f * gx * gy, pxz = 2.f * gx * gz, pyz = 2.f * gy * gz;
+    kf = K * (2.f / 27.f);
+    ke = K * (1.f / 54.f);
+    kc = K * (1.f / 216.f);
+    k0 = K * (10.f / 27.f) * g2;
+    fa[0] = -(y2 + z2);
+    fa[1] = -(x2 + z2);
+    fa[2] = -(x2 + y2);
+    ep[0] = pxy - z2;
+    em[0] = -(pxy + z2);
+    ep[1] = pxz - y2;
+    em[1] = -(pxz + y2);
+    ep[2] = pyz - x2;
+    em[2] = -(pyz + x2);
+    cn[0] = pxy + pxz + pyz;
+    cn[1] = pxy - (pxz + pyz);
+    cn[2] = pxz - (pxy + pyz);
+    cn[3] = pyz - (pxy + pxz);
+  }
   template <int L>
   __device__ __forceinline__ void put(float v) {
-    constexpr float cx = (float)kCX[L], cy = (float)kCY[L], cz = (float)kCZ[L];
-    const float cg = cx * gx + cy * gy + cz * gz;
-    float om = wgt(L) * (cg * cg - g2);
-    if constexpr (L == 0) om += (2.f / 3.f) * g2;
-    in.template put<L>(fmaf(K, om, v));
+    constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+    constexpr int nnz = (cx != 0) + (cy != 0) + (cz != 0);
+    float r;
+    if constexpr (nnz == 0) {
+      r = v + k0;
+    } else if constexpr (nnz == 1) {
+      r = fmaf(kf, fa[cx != 0 ? 0 : (cy != 0 ? 1 : 2)], v);
+    } else if constexpr (nnz == 2) {
+      constexpr int pr = cz == 0 ? 0 : (cy == 0 ? 1 : 2);  // xy, xz, yz
+      constexpr int sab = (cz == 0 ? cx * cy : (cy == 0 ? cx * cz : cy * cz));
+      r = fmaf(ke, sab > 0 ? ep[pr] : em[pr], v);
+    } else {
+      // sign pattern up to a common sign: (sx sy, sx sz) = (+,+) -> 0,
+      // (+,-) -> 1, (-,+) -> 2, (-,-) -> 3
+      constexpr int sxy = cx * cy, sxz = cx * cz;
+      constexpr int k = sxy > 0 ? (sxz > 0 ? 0 : 1) : (sxz > 0 ? 2 : 3);
+      r = fmaf(kc, cn[k], v);
+    }
+    in.template put<L>(r);
   }
   template <int G>
   __device__ __forceinline__ void putg(float) {}
@@ -923,7 +965,7 @@ __device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p,
     const float mg = sqrtf(g2);
     const bool pert = mg > 0.001f;
     const float K = pert ? 2.25f * p.sigma * omega / mg : 0.f;
-    ColourSink<Sink> cs{h, K, gx, gy, gz, g2};
+    ColourSink<Sink> cs(h, K, gx, gy, gz, g2);
     all_pops_mt<ADD>(m, m[1], m[2], m[3], p, cs, 1.f - omega);
     gq[0] = ca;
     gq[1] = pert ? p.beta * ca * cb * (1.f + m[0]) / mg : 0.f;
