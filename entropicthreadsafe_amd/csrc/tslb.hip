@@ -907,16 +907,6 @@ struct ColourSink {
   __device__ __forceinline__ void putg(float) {}
 };
 
-// colour model, second LDS pass: the A population of direction L from the
-// total one (shifted h, as stored) with Latva-Kokko recolouring,
-//   fA_l = cA (h_l + w_l) + bc w_l (c_l.g)/|c_l|,  bc = beta cA cB rho/|g|
-template <int L>
-__device__ __forceinline__ float colour_a(float h, float ca, float bc, float gx, float gy, float gz) {
-  constexpr int nnz = (kCX[L] != 0) + (kCY[L] != 0) + (kCZ[L] != 0);
-  constexpr float inv = nnz == 0 ? 0.f : (nnz == 1 ? 1.f : (nnz == 2 ? 0.70710678118654752f : 0.57735026918962576f));
-  const float cg = (float)kCX[L] * gx + (float)kCY[L] * gy + (float)kCZ[L] * gz;
-  return fmaf(ca, h + wgt(L), bc * (wgt(L) * inv) * cg);
-}
 
 template <int NC, bool ADD, bool CLO, typename Sink>
 __device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p, Sink &h, float *gq) {
@@ -1118,19 +1108,77 @@ __device__ __forceinline__ void bb_layer(Acc &a, const float (*lds)[NSLOT], int 
   acc_layer<CZ>(a, p);
 }
 
+// Colour model, second LDS pass: the A population of each direction from
+// the total one (shifted h, as stored) with Latva-Kokko recolouring,
+//   fA_l = cA (h_l + w_l) + bc w_l (c_l.g)/|c_l|,  bc = beta cA cB rho/|g|,
+// for all 27 directions of one source from shared terms:
+// fA_l = cA h_l + (cA w_l + bc w_l (c_l.g)/|c_l|), where the second part
+// takes 4 values of cA w_l and, up to sign, 13 of the c.g term (faces g_a,
+// edges g_a +- g_b, corners gx +- gy +- gz)
+struct Recolour {
+  float ca, cw0, cwf, cwe, cwc;
+  float tf[3], tep[3], tem[3], tcn[4];
+  __device__ __forceinline__ explicit Recolour(const float *c) {
+    ca = c[0];
+    const float bc = c[1], gx = c[2], gy = c[3], gz = c[4];
+    cw0 = ca * (8.f / 27.f);
+    cwf = ca * (2.f / 27.f);
+    cwe = ca * (1.f / 54.f);
+    cwc = ca * (1.f / 216.f);
+    const float bf = bc * (2.f / 27.f), be = bc * (0.70710678118654752f / 54.f),
+                bn = bc * (0.57735026918962576f / 216.f);
+    tf[0] = bf * gx;
+    tf[1] = bf * gy;
+    tf[2] = bf * gz;
+    tep[0] = be * (gx + gy);
+    tem[0] = be * (gx - gy);
+    tep[1] = be * (gx + gz);
+    tem[1] = be * (gx - gz);
+    tep[2] = be * (gy + gz);
+    tem[2] = be * (gy - gz);
+    const float xpy = gx + gy, xmy = gx - gy;
+    tcn[0] = bn * (xpy + gz);
+    tcn[1] = bn * (xpy - gz);
+    tcn[2] = bn * (xmy + gz);
+    tcn[3] = bn * (xmy - gz);
+  }
+  template <int L>
+  __device__ __forceinline__ float a(float h) const {
+    constexpr int cx = kCX[L], cy = kCY[L], cz = kCZ[L];
+    constexpr int nnz = (cx != 0) + (cy != 0) + (cz != 0);
+    if constexpr (nnz == 0) {
+      return fmaf(ca, h, cw0);
+    } else if constexpr (nnz == 1) {
+      constexpr int ax = cx != 0 ? 0 : (cy != 0 ? 1 : 2);
+      constexpr int sg = cx + cy + cz;
+      return fmaf(ca, h, sg > 0 ? cwf + tf[ax] : cwf - tf[ax]);
+    } else if constexpr (nnz == 2) {
+      // c.g = sa (g_a + sa sb g_b) for the pair (a, b)
+      constexpr int pr = cz == 0 ? 0 : (cy == 0 ? 1 : 2);
+      constexpr int sa = pr == 2 ? cy : cx, sb = pr == 0 ? cy : cz;
+      const float t = sa * sb > 0 ? tep[pr] : tem[pr];
+      return fmaf(ca, h, sa > 0 ? cwe + t : cwe - t);
+    } else {
+      // c.g = sx (gx + sx sy gy + sx sz gz)
+      constexpr int sxy = cx * cy, sxz = cx * cz;
+      constexpr int k = sxy > 0 ? (sxz > 0 ? 0 : 1) : (sxz > 0 ? 2 : 3);
+      return fmaf(ca, h, cx > 0 ? cwc + tcn[k] : cwc - tcn[k]);
+    }
+  }
+};
+
 // colour model: rewrite the 24 in-plane total populations of one LDS slot as
 // A populations (c = {cA, bc, gx, gy, gz})
 template <int L>
-__device__ __forceinline__ void colour_rewrite(float (*lds)[NSLOT], int slot, const float *c) {
+__device__ __forceinline__ void colour_rewrite(float (*lds)[NSLOT], int slot, const Recolour &rc) {
   if constexpr (L < 27) {
-    if constexpr (L != 5 && L != 6)
-      lds[qslot(L)][slot] = colour_a<L>(lds[qslot(L)][slot], c[0], c[1], c[2], c[3], c[4]);
-    colour_rewrite<L + 1>(lds, slot, c);
+    if constexpr (L != 5 && L != 6) lds[qslot(L)][slot] = rc.template a<L>(lds[qslot(L)][slot]);
+    colour_rewrite<L + 1>(lds, slot, rc);
   }
 }
 
 // colour model, a ring source's second pass: its 9 entering total
-// populations become recoloured A populations (colour_a by direction)
+// populations become recoloured A populations (Recolour's, by direction)
 __device__ __forceinline__ void colour_rewrite_ring(float *const (&rq)[3], const float *c, bool col, float sgn) {
   const RingDir d{col, sgn, c[2], c[3], c[4]};
 #pragma unroll
@@ -1525,10 +1573,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
       if (ring_wave) __builtin_amdgcn_s_setprio(1);  // the ring rewrites set this barrier too
       float a0 = 0.f, a5 = 0.f, a6 = 0.f;
       if (co_ok) {
-        colour_rewrite<1>(lds, (ty + 1) * SW + (tx + 1), oc);
-        a0 = colour_a<0>(h0, oc[0], oc[1], oc[2], oc[3], oc[4]);
-        a5 = colour_a<5>(h5, oc[0], oc[1], oc[2], oc[3], oc[4]);
-        a6 = colour_a<6>(h6, oc[0], oc[1], oc[2], oc[3], oc[4]);
+        const Recolour ro(oc);
+        colour_rewrite<1>(lds, (ty + 1) * SW + (tx + 1), ro);
+        a0 = ro.a<0>(h0);
+        a5 = ro.a<5>(h5);
+        a6 = ro.a<6>(h6);
       }
       if (cr_ok) {
         if constexpr (R9) {
@@ -1538,7 +1587,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(WPE, 8))) 
           ring_geom(col, sgn, rq);
           colour_rewrite_ring(rq, rc, col, sgn);
         } else {
-          colour_rewrite<1>(lds, rsy * SW + rsx, rc);
+          colour_rewrite<1>(lds, rsy * SW + rsx, Recolour(rc));
         }
       }
       __builtin_amdgcn_s_setprio(0);
