@@ -134,7 +134,7 @@ struct Params {
   int z_chunk, tiles_x, tiles_y, nzc;
   long long fs;             // element stride between the fields of a state buffer
   int gwall;                // 1: walls absorb phase populations (jet boundary)
-  float nua, nub;           // colour model: viscosities of components A and B
+  float inua, inub;         // colour model: 1/nu of components A and B
   float beta;               // colour model: recolouring parameter
   float tau;                // relaxation time (learned closure: omega = 1/(tau + 3 nu_t))
   float sc[4];   // storage scale per field group, sgrp(q) (1 for FP32)
@@ -908,6 +908,30 @@ struct ColourSink {
 };
 
 
+// Colour model, per node (recursive_colorTSLB3D.f90:719-959, corrected):
+// omega from the harmonic-mean viscosity of the two components (:756-758),
+// 1/nu = cA/nuA + cB/nuB = x and omega = 1/(3 nu + 1/2) = x/(3 + x/2); the
+// perturbation strength K = (9/4) sigma omega/|g| and the recolouring factor
+// bc = beta cA cB rho/|g| where |g| > 0.001 (0 elsewhere).  Hardware
+// reciprocal and reciprocal square root (one ulp) in place of four
+// correctly rounded divisions and a square root.
+struct ColourNode {
+  float ca, omega, K, bc, g2;
+};
+__device__ __forceinline__ ColourNode colour_node(const float *m, float gx, float gy, float gz, const Params &p) {
+  ColourNode c;
+  c.ca = fminf(fmaxf(0.5f * (1.f + m[10]), 0.f), 1.f);
+  const float cb = 1.f - c.ca;
+  const float x = fminf(fmaf(c.ca, p.inua, cb * p.inub), 3e30f);  // nu = 0 (tau = 1/2): omega = 2
+  c.omega = x * __builtin_amdgcn_rcpf(fmaf(0.5f, x, 3.f));
+  c.g2 = fmaf(gx, gx, fmaf(gy, gy, gz * gz));
+  const bool pert = c.g2 > 1e-6f;
+  const float ig = __builtin_amdgcn_rsqf(c.g2);
+  c.K = pert ? 2.25f * p.sigma * c.omega * ig : 0.f;
+  c.bc = pert ? p.beta * c.ca * cb * (1.f + m[0]) * ig : 0.f;
+  return c;
+}
+
 template <int NC, bool ADD, bool CLO, typename Sink>
 __device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p, Sink &h, float *gq) {
   const float *m = s.m;
@@ -943,22 +967,13 @@ __device__ __forceinline__ void eval_src(const Src<NC, CLO> &s, const Params &p,
     gq[5] = base + az;
     gq[6] = base - az;
   } else if constexpr (NC == 3) {
-    // colour model (recursive_colorTSLB3D.f90:719-959, corrected): per-node
-    // omega from the harmonic-mean viscosity of the two components (:756-758)
-    float ca = 0.5f * (1.f + m[10]);
-    ca = fminf(fmaxf(ca, 0.f), 1.f);
-    const float cb = 1.f - ca;
-    const float nu = 1.f / (ca / p.nua + cb / p.nub);
-    const float omega = 1.f / fmaf(3.f, nu, 0.5f);
+    // colour model: per-node omega, perturbation and recolouring factor
     const float gx = s.nn[0], gy = s.nn[1], gz = s.nn[2];
-    const float g2 = gx * gx + gy * gy + gz * gz;
-    const float mg = sqrtf(g2);
-    const bool pert = mg > 0.001f;
-    const float K = pert ? 2.25f * p.sigma * omega / mg : 0.f;
-    ColourSink<Sink> cs(h, K, gx, gy, gz, g2);
-    all_pops_mt<ADD>(m, m[1], m[2], m[3], p, cs, 1.f - omega);
-    gq[0] = ca;
-    gq[1] = pert ? p.beta * ca * cb * (1.f + m[0]) / mg : 0.f;
+    const ColourNode c = colour_node(m, gx, gy, gz, p);
+    ColourSink<Sink> cs(h, c.K, gx, gy, gz, c.g2);
+    all_pops_mt<ADD>(m, m[1], m[2], m[3], p, cs, 1.f - c.omega);
+    gq[0] = c.ca;
+    gq[1] = c.bc;
     gq[2] = gx;
     gq[3] = gy;
     gq[4] = gz;
@@ -984,20 +999,12 @@ __device__ __forceinline__ void eval_ring(const Src<NC, CLO> &s, const Params &p
   if constexpr (NC == 3) {
     // the colour model (eval_src's, on the 9 entering populations); rf
     // returns cA, bc, gx, gy, gz for the recolouring pass
-    float ca = 0.5f * (1.f + m[10]);
-    ca = fminf(fmaxf(ca, 0.f), 1.f);
-    const float cb = 1.f - ca;
-    const float nu = 1.f / (ca / p.nua + cb / p.nub);
-    const float omega = 1.f / fmaf(3.f, nu, 0.5f);
     const float gx = s.nn[0], gy = s.nn[1], gz = s.nn[2];
-    const float g2 = gx * gx + gy * gy + gz * gz;
-    const float mg = sqrtf(g2);
-    const bool pert = mg > 0.001f;
-    const float K = pert ? 2.25f * p.sigma * omega / mg : 0.f;
-    const ColourRingAdj adj{RingDir{col, sgn, gx, gy, gz}, K, g2};
-    ring_pops<ADD>(m, m[1], m[2], m[3], p, 1.f - omega, col, sgn, rq, adj);
-    rf[0] = ca;
-    rf[1] = pert ? p.beta * ca * cb * (1.f + m[0]) / mg : 0.f;
+    const ColourNode c = colour_node(m, gx, gy, gz, p);
+    const ColourRingAdj adj{RingDir{col, sgn, gx, gy, gz}, c.K, c.g2};
+    ring_pops<ADD>(m, m[1], m[2], m[3], p, 1.f - c.omega, col, sgn, rq, adj);
+    rf[0] = c.ca;
+    rf[1] = c.bc;
     rf[2] = gx;
     rf[3] = gy;
     rf[4] = gz;
@@ -3038,8 +3045,8 @@ int tslb_create(tslb_ctx **out, const tslb_params *p) {
   k.ncomp = p->ncomp;
   // the colour model takes its body force additively (recursive_colorTSLB3D.f90's fx*p1dcssq)
   k.shift = (p->ncomp == 2 || (p->ncomp == 1 && p->force_mode == TSLB_FORCE_SHIFT)) ? 1 : 0;
-  k.nua = (p->tau - 0.5f) / 3.f;
-  k.nub = ((p->tau_b > 0.5f ? p->tau_b : p->tau) - 0.5f) / 3.f;
+  k.inua = 3.f / (p->tau - 0.5f);
+  k.inub = 3.f / ((p->tau_b > 0.5f ? p->tau_b : p->tau) - 0.5f);
   k.beta = p->beta;
   k.om1 = 1.f - 1.f / p->tau;
   k.tau = p->tau;
