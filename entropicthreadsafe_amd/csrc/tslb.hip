@@ -832,7 +832,7 @@ __device__ __forceinline__ const float *closure_moments(const Src<NC, CLO> &s, c
     om1 = p.om1;
     return s.m;
   } else {
-    const float o1 = 1.f - 1.f / fmaf(3.f, s.clo[0], p.tau);
+    const float o1 = 1.f - __builtin_amdgcn_rcpf(fmaf(3.f, s.clo[0], p.tau));  // one ulp
 #pragma unroll
     for (int q = 0; q < 4; q++) buf[q] = s.m[q];
 #pragma unroll
